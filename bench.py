@@ -1,0 +1,188 @@
+"""Benchmark of the POA alignment path on one MI355X.
+
+Workload (default): seeded windows of the convex per-call shape -- a 1.5 kb
+uniform ACGT template, 30 reads of 1-2 kb at 15% mixed error, local
+alignment with m=5 n=-4 g=-8 e=-6 q=-10 c=-4 -- every read aligned
+(Align) and merged (AddAlignment) in order, then GenerateConsensus and the
+MSA.  One step runs the whole batch of windows through the batched form
+(one persistent launch per round of reads).  Alongside, on the same bytes:
+
+* the plain per-call drop-in (AlignmentEngine::Align, one read at a time)
+  on one window,
+* the oracle (the repository's CPU restatement of the reference) on the
+  host cores on a bounded sample of windows, as the CPU baseline.
+
+Prints one JSON line.  Inputs are resident on the host when a step starts
+and every step ends with all device work complete (the window driver
+synchronises its stream), so the timed region brackets whole-job work.
+"""
+import argparse
+import concurrent.futures
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import spoa_amd  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_baseline(windows, params, strand, budget_s, threads):
+    """Times the oracle on the host cores over as many windows as fit about
+    budget_s seconds (at least one per thread).  Returns a cpu_baseline
+    record, or None where the oracle library is absent."""
+    so = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(so):
+        return None
+    lib = ctypes.CDLL(so)
+    fn = lib.oracle_run_window
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int] * 7 + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int,
+                                        ctypes.c_char_p, ctypes.c_uint32]
+    full = tuple(params) + (None,) * (7 - len(params))
+    t, m, n, g, e, q, c = full
+    e = g if e is None else e
+    if q is None:
+        q, c = g, e
+
+    def one(w):
+        reads = windows.reads(w)
+        data = "".join(reads).encode()
+        offs = [0]
+        for r in reads:
+            offs.append(offs[-1] + len(r))
+        arr = (ctypes.c_uint64 * len(offs))(*offs)
+        out = ctypes.create_string_buffer(1 << 20)
+        k = fn(t, m, n, g, e, q, c, len(reads), arr, data, int(strand), out, 1 << 20)
+        return out.raw[:k].decode() if k >= 0 else None
+
+    # one window per thread first, to size the sample to the budget
+    t0 = time.perf_counter()
+    done = 0
+    results = {}
+    with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
+        nxt = 0
+        futs = {}
+        while nxt < windows.n_windows and len(futs) < threads:
+            futs[ex.submit(one, nxt)] = nxt
+            nxt += 1
+        while futs:
+            fin, _ = concurrent.futures.wait(list(futs), return_when=concurrent.futures.FIRST_COMPLETED)
+            for f in fin:
+                results[futs.pop(f)] = f.result()
+                done += 1
+            if time.perf_counter() - t0 < budget_s and nxt < windows.n_windows:
+                while nxt < windows.n_windows and len(futs) < threads:
+                    futs[ex.submit(one, nxt)] = nxt
+                    nxt += 1
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "windows/s", "cores": threads, "kind": "port",
+            "sample": "%d windows of the benchmark workload through the oracle (C++ restatement of the "
+                      "reference, scalar) on %d host threads, %.1f s" % (done, threads, el),
+            "consensus": results}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4])
+    ap.add_argument("--windows", type=int, default=None, help="windows per step (default per config)")
+    ap.add_argument("--seed", type=int, default=20240601)
+    ap.add_argument("--threads", type=int, default=0, help="host threads (0: all, at most 16)")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-per-call", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+              "(windows shard across GPUs without any exchange, see DESIGN.md)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    cfg = spoa_amd.CONFIGS[args.config]
+    n_windows = args.windows or {1: 1024, 2: 512, 3: 4096, 4: 64}[args.config]
+    threads = args.threads or min(16, os.cpu_count() or 1)
+    params = cfg["params"]
+    strand = cfg.get("strand", False)
+    windows = spoa_amd.config_windows(args.config, args.seed, n_windows)
+
+    def step():
+        return spoa_amd.run_windows(windows, *params, mode="batched", strand=strand, msa=True,
+                                    threads=threads)
+
+    for _ in range(args.warmup):
+        step()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+        stats.append(res.stats)
+    el = time.perf_counter() - t0
+    ms_per_step = 1000.0 * el / args.steps
+    wps = n_windows * args.steps / el
+    cells = sum(s["cells"] for s in stats)
+    dp_bytes = sum(s["dp_bytes"] for s in stats)
+    kernel_ms = sum(s["kernel_ms"] for s in stats)
+    launches = sum(s["launches"] for s in stats)
+    achieved = dp_bytes / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else 0.0
+
+    # plain per-call drop-in on the first window (same bytes)
+    per_call = None
+    if not args.no_per_call:
+        one = windows.subset([0])
+        t1 = time.perf_counter()
+        pc = spoa_amd.run_windows(one, *params, mode="per_call", strand=strand, msa=True)
+        pel = time.perf_counter() - t1
+        per_call = {"windows_per_s": 1.0 / pel, "ms_per_window": 1000.0 * pel,
+                    "gcups": pc.stats["cells"] / (pel * 1e9),
+                    "kernel_ms": pc.stats["kernel_ms"], "align_calls": pc.stats["align_calls"],
+                    "same_consensus_as_batched": pc.consensus[0] == res.consensus[0]}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(windows, params, strand, args.cpu_budget, threads)
+        if cpu is not None:
+            agree = all(res.consensus[w] == c for w, c in cpu.pop("consensus").items())
+            cpu["same_consensus_as_gpu"] = agree
+
+    last = stats[-1]
+    out = {
+        "metric": "windows/s (batched GPU Align + AddAlignment + consensus + MSA)",
+        "value": wps,
+        "unit": "windows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16/int32",
+        "data": "synthetic (seeded uniform ACGT templates, mixed substitution/insertion/deletion errors)",
+        "config": {"workload": cfg["name"], "windows_per_step": n_windows, "seed": args.seed,
+                   "params": list(params), "strand": strand, "host_threads": threads},
+        "gcups_dp": cells / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else 0.0,
+        "gcups_end_to_end": cells / el / 1e9,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "PoaAlignKernel (batched launches)", "launches": launches,
+                     "avg_launch_ms": kernel_ms / max(1, launches),
+                     "bytes_per_launch": dp_bytes / max(1, launches)},
+        "breakdown_ms_per_step": {k: last[k] for k in ("wall_ms", "align_ms", "kernel_ms", "h2d_ms",
+                                                       "d2h_ms", "graph_ms", "finish_ms")},
+        "per_call": per_call,
+        "batched_vs_per_call": (wps / per_call["windows_per_s"]) if per_call else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,107 @@
+/*
+ * spoa_capi.h -- C entry points over the C++ drop-in API (spoa::AlignmentEngine,
+ * spoa::Graph, include/spoa/spoa.hpp), the batched and per-call window
+ * drivers, and the seeded synthetic window generator.  For callers that
+ * bind C (the Python package binds these with ctypes).
+ *
+ * Errors: functions returning int return a negative value (-1: the C++ call
+ * threw, message in spoa_last_error(); -2: an output buffer is too small),
+ * functions returning a handle return NULL.
+ */
+#ifndef SPOA_CAPI_H_
+#define SPOA_CAPI_H_
+
+#include <stdint.h>
+
+#include "spoa_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* spoa_last_error(void);
+
+/* AlignmentEngine::Create(type, m, n, g, e, q, c) (reference
+ * include/spoa/alignment_engine.hpp:49-55); the engine runs on the GPU. */
+void* spoa_engine_new(int type, int m, int n, int g, int e, int q, int c);
+void spoa_engine_free(void* engine);
+int spoa_engine_subtype(void* engine);
+/* AlignmentEngine::Prealloc (alignment_engine.hpp:57-59) */
+int spoa_engine_prealloc(void* engine, uint32_t max_len, uint32_t alphabet);
+/* measurements of the engine's last Align */
+int spoa_engine_stats(void* engine, spoa_hip_stats* out);
+/* the C ABI handle behind an engine (tuning, memory budget) */
+spoa_hip_engine* spoa_engine_device(void* engine);
+/* AlignmentEngine::Align (alignment_engine.hpp:61-69): returns the number of
+ * pairs written to pairs_out (2 int32 each); *score / *has_score as the
+ * reference writes its score; *width_bits the score width used. */
+int spoa_align(void* engine, void* graph, const char* seq, uint32_t len, int32_t* pairs_out, uint32_t cap,
+               int32_t* score, int* has_score, int* width_bits);
+
+/* spoa::Graph (reference include/spoa/graph.hpp:26-318) */
+void* spoa_graph_new(void);
+void spoa_graph_free(void* graph);
+void spoa_graph_clear(void* graph);
+uint32_t spoa_graph_num_nodes(void* graph);
+/* Graph::AddAlignment with per-base weights (graph.hpp:154-157), or weight 1
+ * when weights is NULL (graph.hpp:149-152) */
+int spoa_graph_add_alignment(void* graph, const int32_t* pairs, uint32_t n_pairs, const char* seq, uint32_t len,
+                             const uint32_t* weights, uint32_t n_weights);
+/* Graph::GenerateConsensus (graph.hpp:167); returns the length */
+int spoa_graph_consensus(void* graph, char* out, uint32_t cap);
+/* Graph::GenerateMultipleSequenceAlignment (graph.hpp:164-165): rows back to back */
+int spoa_graph_msa(void* graph, int include_consensus, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols);
+/* rank_to_node() ids (graph.hpp:110-112) */
+int spoa_graph_ranks(void* graph, uint32_t* out, uint32_t cap);
+
+/* ----------------------------------------------------------- windows -- */
+enum { SPOA_BATCH_BATCHED = 0, SPOA_BATCH_PER_CALL = 1 };
+
+typedef struct spoa_batch_opts {
+  int32_t type, m, n, g, e, q, c;
+  int32_t strand;  /* align both strands and keep the better (reference src/main.cpp:285-302) */
+  int32_t msa;     /* also build every window's MSA */
+  int32_t threads; /* host threads, 0 = all */
+  int32_t device;
+} spoa_batch_opts;
+
+typedef struct spoa_batch_stats {
+  double wall_ms, align_ms, kernel_ms, h2d_ms, d2h_ms, graph_ms, finish_ms;
+  uint64_t cells, dp_bytes, align_calls;
+  uint32_t launches, rounds;
+} spoa_batch_stats;
+
+/* Runs every window (reads [win_read_off[w], win_read_off[w+1]); read k is
+ * seqs[read_off[k], read_off[k+1]), quals likewise or NULL) through
+ * Align + AddAlignment in order, then GenerateConsensus (+ MSA).
+ * mode SPOA_BATCH_BATCHED: one launch per round over all live windows;
+ * SPOA_BATCH_PER_CALL: the drop-in Align, one read at a time. */
+void* spoa_batch_run(int mode, const spoa_batch_opts* opts, uint32_t n_windows, const uint32_t* win_read_off,
+                     const uint64_t* read_off, const char* seqs, const char* quals);
+void spoa_batch_free(void* result);
+int spoa_batch_get_stats(void* result, spoa_batch_stats* out);
+int spoa_batch_consensus(void* result, uint32_t window, char* out, uint32_t cap);
+int spoa_batch_msa(void* result, uint32_t window, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols);
+int spoa_batch_reversed(void* result, uint32_t window, uint8_t* out, uint32_t cap);
+
+/* ------------------------------------------------------------- synth -- */
+typedef struct spoa_synth_spec {
+  uint32_t template_min, template_max; /* template length range */
+  uint32_t reads_min, reads_max;       /* reads per window */
+  uint32_t read_len_min, read_len_max; /* 0, 0: every read covers the template */
+  double err_min, err_max;             /* per-window error rate range */
+  uint32_t revcomp_odd;                /* odd-numbered reads reverse-complemented */
+  uint32_t log_uniform_len;            /* read length log-uniform in its range */
+} spoa_synth_spec;
+
+void* spoa_synth_new(const spoa_synth_spec* spec, uint64_t seed, uint32_t n_windows);
+void spoa_synth_free(void* h);
+void spoa_synth_sizes(void* h, uint32_t* n_windows, uint64_t* n_reads, uint64_t* seq_bytes, uint64_t* templ_bytes);
+void spoa_synth_copy(void* h, uint32_t* win_read_off, uint64_t* read_off, char* seqs, uint64_t* templ_off,
+                     char* templates);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPOA_CAPI_H_ */

@@ -1,0 +1,153 @@
+/*
+ * spoa_hip.h -- the C ABI between the C++ host code (spoa::AlignmentEngine,
+ * the batched window driver, the serving loop) and the HIP graph-to-sequence
+ * alignment kernels for gfx950.
+ *
+ * Plain pointers and sizes only; every host buffer is owned by the caller.
+ * Every entry point returns a status code (SPOA_HIP_OK on success); the
+ * message of the last failure on the calling thread is returned by
+ * spoa_hip_last_error().  Messages of the conditions the reference reports
+ * are the reference's own strings.
+ *
+ * The graph is handed over as a rank-ordered CSR view of spoa::Graph:
+ * rank_to_node() order, Node::inedges order of the predecessors,
+ * Node::outedges emptiness and decoder(Node::code) (reference
+ * include/spoa/graph.hpp:102-132).
+ */
+#ifndef SPOA_HIP_H_
+#define SPOA_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum spoa_hip_status {
+  SPOA_HIP_OK = 0,
+  SPOA_HIP_INVALID_ARGUMENT = 1, /* bad parameter (the message says which) */
+  SPOA_HIP_TOO_LARGE = 2,        /* "too large sequence!" */
+  SPOA_HIP_OVERFLOW = 3,         /* "possible overflow!" */
+  SPOA_HIP_NO_MEMORY = 4,        /* "insufficient memory!" */
+  SPOA_HIP_DEVICE_ERROR = 5,     /* a HIP runtime call failed */
+  SPOA_HIP_CAPACITY = 6,         /* an output buffer is too small */
+  SPOA_HIP_INCONSISTENT = 7      /* the traceback found no consistent move */
+};
+
+/* Alignment type and gap subtype, numbered as spoa::AlignmentType
+ * (reference include/spoa/alignment_engine.hpp:16-20) and
+ * spoa::AlignmentSubtype (alignment_engine.hpp:22-26). */
+enum spoa_hip_type { SPOA_HIP_SW = 0, SPOA_HIP_NW = 1, SPOA_HIP_OV = 2 };
+enum spoa_hip_subtype { SPOA_HIP_LINEAR = 0, SPOA_HIP_AFFINE = 1, SPOA_HIP_CONVEX = 2 };
+
+/* Scoring parameters after AlignmentEngine::Create's validation and
+ * normalisation (reference src/alignment_engine.cpp:32-66): linear has
+ * e == g, affine has q == g and c == e. */
+typedef struct spoa_hip_params {
+  int32_t type;
+  int32_t subtype;
+  int8_t m, n, g, e, q, c;
+} spoa_hip_params;
+
+/* Rank-ordered view of a POA graph with V nodes. */
+typedef struct spoa_hip_graph {
+  uint32_t num_nodes;        /* V */
+  const uint8_t* row_char;   /* [V] decoder(code) of the node at each rank */
+  const int32_t* node_id;    /* [V] node id at each rank */
+  const uint32_t* pred_off;  /* [V + 1] CSR offsets into pred_rank */
+  const uint32_t* pred_rank; /* predecessor ranks, in Node::inedges order */
+  const uint8_t* is_sink;    /* [V] 1 if the node has no out-edges */
+} spoa_hip_graph;
+
+/* One alignment of a batch. */
+typedef struct spoa_hip_job {
+  spoa_hip_graph graph;
+  const char* seq;
+  uint32_t len;
+  int32_t* pairs;      /* out: 2 * cap int32, (node id, column) pairs in path order */
+  uint32_t cap;        /* capacity of pairs, in pairs (V + len suffices) */
+  uint32_t n_pairs;    /* out */
+  int32_t score;       /* out: valid when has_score != 0 */
+  int32_t has_score;   /* out: 0 where the reference leaves *score untouched */
+  int32_t width_bits;  /* out: 16 or 32 (the score width the range bound chose), 0 if no DP ran */
+  int32_t status;      /* out: per-job status */
+} spoa_hip_job;
+
+typedef struct spoa_hip_engine spoa_hip_engine;
+
+/* Replaces the device half of AlignmentEngine::Create (reference
+ * include/spoa/alignment_engine.hpp:35-55, src/alignment_engine.cpp:32-73):
+ * binds an engine with normalised parameters to a HIP device. */
+int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_engine** out);
+
+int spoa_hip_engine_destroy(spoa_hip_engine* engine);
+
+/* Replaces AlignmentEngine::Prealloc (reference alignment_engine.hpp:57-59,
+ * src/simd_alignment_engine_implementation.hpp:381-422): reserves device
+ * memory for reads up to max_len over graphs of max_len * alphabet_size
+ * nodes.  Errors: SPOA_HIP_TOO_LARGE, SPOA_HIP_NO_MEMORY. */
+int spoa_hip_prealloc(spoa_hip_engine* engine, uint32_t max_len, uint32_t alphabet_size);
+
+/* Replaces AlignmentEngine::Align (reference alignment_engine.hpp:66-69,
+ * src/simd_alignment_engine_implementation.hpp:703-2131): aligns one read
+ * to the graph on the device (DP + traceback) and returns the
+ * (node id, column) pairs in path order; *score and *has_score as the
+ * reference writes its score.  Errors: SPOA_HIP_TOO_LARGE,
+ * SPOA_HIP_OVERFLOW, SPOA_HIP_NO_MEMORY, SPOA_HIP_CAPACITY. */
+int spoa_hip_align(spoa_hip_engine* engine, const spoa_hip_graph* graph, const char* seq, uint32_t len,
+                   int32_t* pairs, uint32_t cap, uint32_t* n_pairs, int32_t* score, int32_t* has_score);
+
+/* No counterpart in the reference, which aligns one read per call: aligns
+ * n_jobs independent (graph, read) pairs, one persistent launch per launch
+ * configuration (and per memory-budget chunk), jobs queued by V x n.
+ * Per-job results and status are written into the job records; the return
+ * value is the first failure. */
+int spoa_hip_align_batch(spoa_hip_engine* engine, spoa_hip_job* jobs, uint32_t n_jobs);
+
+/* Measurements of the last align / align_batch call: device time of the
+ * alignment kernels from HIP events on the engine's stream, copies, launch
+ * count, DP cells (sum of V * n) and algorithmic DP bytes (sum of
+ * S * n * (W * V + Rd * R): every score matrix written once, the
+ * predecessor rows read once per in-edge). */
+typedef struct spoa_hip_stats {
+  double kernel_ms;
+  double h2d_ms;
+  double d2h_ms;
+  double host_ms;
+  uint32_t launches;
+  uint32_t jobs;
+  uint64_t cells;
+  uint64_t dp_bytes;
+} spoa_hip_stats;
+int spoa_hip_last_stats(spoa_hip_engine* engine, spoa_hip_stats* out);
+
+/* Launch configuration (columns per lane, waves per workgroup) for a read
+ * of length len at a score width; batch != 0 asks for the throughput
+ * configuration of align_batch, 0 for the latency one of align. */
+int spoa_hip_config(spoa_hip_engine* engine, uint32_t len, int32_t width_bits, int32_t batch,
+                    int32_t* cols_per_lane, int32_t* waves);
+
+/* Override the launch configuration for rows whose reference width
+ * (ceil(len / L) * L) is at most max_width, at one score width and mode;
+ * used by the autotuner.  cols_per_lane in {8, 16} (convex int16 needs 16),
+ * waves in 1..16.  cols_per_lane == 0 clears all overrides of that width
+ * and mode. */
+int spoa_hip_set_config(spoa_hip_engine* engine, int32_t width_bits, int32_t batch, uint32_t max_width,
+                        int32_t cols_per_lane, int32_t waves);
+
+/* Device memory budget for DP matrices (bytes).  A batch whose matrices do
+ * not fit is split into launches that do; a single job that does not fit
+ * fails with SPOA_HIP_NO_MEMORY ("insufficient memory!"). */
+int spoa_hip_set_memory_budget(spoa_hip_engine* engine, uint64_t bytes);
+
+/* DP-matrix bytes one job needs (the admission footprint used by the
+ * serving loop), or 0 when the job runs no DP. */
+uint64_t spoa_hip_job_footprint(spoa_hip_engine* engine, uint32_t num_nodes, uint32_t len, int32_t batch);
+
+const char* spoa_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPOA_HIP_H_ */

@@ -1,0 +1,126 @@
+// TEST INFRASTRUCTURE ONLY.
+//
+// A C shim over the reference spoa library (built from the sources under
+// /root/reference by oracle/Makefile into oracle/_ref/).  It exposes the
+// same entry points as the oracle (ref_* instead of oracle_*), so tests can
+// drive the reference, the oracle and the product through one harness.
+// Bound reference interfaces: AlignmentEngine::Create / Align
+// (include/spoa/alignment_engine.hpp:35-69), Graph::AddAlignment /
+// GenerateConsensus / GenerateMultipleSequenceAlignment / rank_to_node
+// (include/spoa/graph.hpp:106-171).
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "spoa/spoa.hpp"
+
+namespace {
+thread_local std::string g_error;
+}
+
+extern "C" {
+
+const char* ref_last_error() { return g_error.c_str(); }
+
+void* ref_engine_new(int type, int m, int n, int g, int e, int q, int c) {
+  try {
+    return spoa::AlignmentEngine::Create(static_cast<spoa::AlignmentType>(type), m, n, g, e, q, c)
+        .release();
+  } catch (std::exception& ex) {
+    g_error = ex.what();
+    return nullptr;
+  }
+}
+
+void ref_engine_free(void* p) { delete static_cast<spoa::AlignmentEngine*>(p); }
+
+int ref_engine_prealloc(void* p, std::uint32_t max_len, std::uint32_t alphabet) {
+  try {
+    static_cast<spoa::AlignmentEngine*>(p)->Prealloc(max_len, static_cast<std::uint8_t>(alphabet));
+    return 0;
+  } catch (std::exception& ex) {
+    g_error = ex.what();
+    return -1;
+  }
+}
+
+void* ref_graph_new() { return new spoa::Graph(); }
+void ref_graph_free(void* p) { delete static_cast<spoa::Graph*>(p); }
+void ref_graph_clear(void* p) { static_cast<spoa::Graph*>(p)->Clear(); }
+std::uint32_t ref_graph_num_nodes(void* p) {
+  return static_cast<std::uint32_t>(static_cast<spoa::Graph*>(p)->nodes().size());
+}
+
+int ref_align(void* eng, void* graph, const char* seq, std::uint32_t len,
+              std::int32_t* pairs_out, std::uint32_t cap, std::int32_t* score,
+              int* has_score, int* width_bits) {
+  try {
+    const std::int32_t kUnset = INT32_MIN;
+    std::int32_t s = kUnset;
+    auto a = static_cast<spoa::AlignmentEngine*>(eng)->Align(
+        seq, len, *static_cast<spoa::Graph*>(graph), &s);
+    if (a.size() > cap) return -2;
+    for (std::size_t k = 0; k < a.size(); ++k) {
+      pairs_out[2 * k] = a[k].first;
+      pairs_out[2 * k + 1] = a[k].second;
+    }
+    if (has_score) *has_score = s != kUnset ? 1 : 0;
+    if (s != kUnset && score) *score = s;
+    if (width_bits) *width_bits = 0;  // not observable through the public API
+    return static_cast<int>(a.size());
+  } catch (std::exception& ex) {
+    g_error = ex.what();
+    return -1;
+  }
+}
+
+int ref_graph_add_alignment(void* graph, const std::int32_t* pairs, std::uint32_t n_pairs,
+                            const char* seq, std::uint32_t len,
+                            const std::uint32_t* weights, std::uint32_t n_weights) {
+  try {
+    spoa::Alignment a(n_pairs);
+    for (std::uint32_t k = 0; k < n_pairs; ++k) a[k] = {pairs[2 * k], pairs[2 * k + 1]};
+    auto& g = *static_cast<spoa::Graph*>(graph);
+    if (weights) {
+      std::vector<std::uint32_t> w(weights, weights + n_weights);
+      g.AddAlignment(a, seq, len, w);
+    } else {
+      g.AddAlignment(a, seq, len);
+    }
+    return 0;
+  } catch (std::exception& ex) {
+    g_error = ex.what();
+    return -1;
+  }
+}
+
+int ref_graph_consensus(void* graph, char* out, std::uint32_t cap) {
+  std::string s = static_cast<spoa::Graph*>(graph)->GenerateConsensus();
+  if (s.size() > cap) return -2;
+  std::memcpy(out, s.data(), s.size());
+  return static_cast<int>(s.size());
+}
+
+int ref_graph_msa(void* graph, int include_consensus, char* out, std::uint64_t cap,
+                  std::uint32_t* rows, std::uint32_t* cols) {
+  auto m = static_cast<spoa::Graph*>(graph)->GenerateMultipleSequenceAlignment(include_consensus != 0);
+  *rows = static_cast<std::uint32_t>(m.size());
+  *cols = m.empty() ? 0 : static_cast<std::uint32_t>(m[0].size());
+  std::uint64_t need = static_cast<std::uint64_t>(*rows) * *cols;
+  if (need > cap) return -2;
+  for (std::size_t r = 0; r < m.size(); ++r) std::memcpy(out + r * (*cols), m[r].data(), *cols);
+  return 0;
+}
+
+int ref_graph_ranks(void* graph, std::uint32_t* out, std::uint32_t cap) {
+  const auto& r = static_cast<spoa::Graph*>(graph)->rank_to_node();
+  if (r.size() > cap) return -2;
+  for (std::size_t k = 0; k < r.size(); ++k) out[k] = r[k]->id;
+  return static_cast<int>(r.size());
+}
+
+}  // extern "C"

@@ -1,0 +1,418 @@
+"""spoa_amd -- graph-to-sequence alignment (partial order alignment) on the
+MI355X, behind spoa's own interface.
+
+The C++ drop-in (``include/spoa/spoa.hpp``: ``spoa::AlignmentEngine``,
+``spoa::Graph``) is compiled into ``libspoa_amd.so`` together with the HIP
+kernels for gfx950; this module binds its C entry points
+(``include/spoa_capi.h``) with ctypes and mirrors the reference interface:
+
+* ``AlignmentEngine.create(type, m, n, g[, e[, q, c]])`` -- reference
+  include/spoa/alignment_engine.hpp:35-55, same validation and messages;
+* ``engine.align(sequence, graph)`` -- alignment_engine.hpp:61-69, returns
+  the (node id, position) pairs and the score;
+* ``Graph.add_alignment`` / ``generate_consensus`` /
+  ``generate_multiple_sequence_alignment`` -- include/spoa/graph.hpp:134-171.
+
+Beyond the reference: ``run_windows`` aligns many independent windows in
+the batched form (one persistent launch per round) or in the plain per-call
+form, and ``synth_windows`` produces the seeded synthetic windows the tests
+and the benchmark use.  There is no CPU fallback: without the built library
+or without a GPU every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+SW, NW, OV = 0, 1, 2
+LINEAR, AFFINE, CONVEX = 0, 1, 2
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspoa_amd.so")
+
+_lib = None
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class HipStats(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double), ("h2d_ms", ctypes.c_double),
+                ("d2h_ms", ctypes.c_double), ("host_ms", ctypes.c_double),
+                ("launches", ctypes.c_uint32), ("jobs", ctypes.c_uint32),
+                ("cells", ctypes.c_uint64), ("dp_bytes", ctypes.c_uint64)]
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in
+                ("type", "m", "n", "g", "e", "q", "c", "strand", "msa", "threads", "device")]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("wall_ms", ctypes.c_double), ("align_ms", ctypes.c_double),
+                ("kernel_ms", ctypes.c_double), ("h2d_ms", ctypes.c_double),
+                ("d2h_ms", ctypes.c_double), ("graph_ms", ctypes.c_double),
+                ("finish_ms", ctypes.c_double), ("cells", ctypes.c_uint64),
+                ("dp_bytes", ctypes.c_uint64), ("align_calls", ctypes.c_uint64),
+                ("launches", ctypes.c_uint32), ("rounds", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SynthSpec(ctypes.Structure):
+    _fields_ = [("template_min", ctypes.c_uint32), ("template_max", ctypes.c_uint32),
+                ("reads_min", ctypes.c_uint32), ("reads_max", ctypes.c_uint32),
+                ("read_len_min", ctypes.c_uint32), ("read_len_max", ctypes.c_uint32),
+                ("err_min", ctypes.c_double), ("err_max", ctypes.c_double),
+                ("revcomp_odd", ctypes.c_uint32), ("log_uniform_len", ctypes.c_uint32)]
+
+
+def lib():
+    """The native library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "spoa_amd: %s is missing; build it with __graft_entry__.build() "
+            "(make -C spoa_amd)" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    L.spoa_last_error.restype = ctypes.c_char_p
+    L.spoa_engine_new.restype = ctypes.c_void_p
+    L.spoa_engine_new.argtypes = [ctypes.c_int] * 7
+    L.spoa_engine_free.argtypes = [ctypes.c_void_p]
+    L.spoa_engine_subtype.argtypes = [ctypes.c_void_p]
+    L.spoa_engine_prealloc.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
+    L.spoa_engine_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(HipStats)]
+    L.spoa_engine_device.restype = ctypes.c_void_p
+    L.spoa_engine_device.argtypes = [ctypes.c_void_p]
+    L.spoa_align.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32,
+                             _i32p, ctypes.c_uint32, _i32p, ctypes.POINTER(ctypes.c_int),
+                             ctypes.POINTER(ctypes.c_int)]
+    L.spoa_graph_new.restype = ctypes.c_void_p
+    L.spoa_graph_free.argtypes = [ctypes.c_void_p]
+    L.spoa_graph_clear.argtypes = [ctypes.c_void_p]
+    L.spoa_graph_num_nodes.restype = ctypes.c_uint32
+    L.spoa_graph_num_nodes.argtypes = [ctypes.c_void_p]
+    L.spoa_graph_add_alignment.argtypes = [ctypes.c_void_p, _i32p, ctypes.c_uint32, ctypes.c_char_p,
+                                           ctypes.c_uint32, _u32p, ctypes.c_uint32]
+    L.spoa_graph_consensus.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32]
+    L.spoa_graph_msa.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint64,
+                                 _u32p, _u32p]
+    L.spoa_graph_ranks.argtypes = [ctypes.c_void_p, _u32p, ctypes.c_uint32]
+    L.spoa_batch_run.restype = ctypes.c_void_p
+    L.spoa_batch_run.argtypes = [ctypes.c_int, ctypes.POINTER(BatchOpts), ctypes.c_uint32,
+                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]
+    L.spoa_batch_free.argtypes = [ctypes.c_void_p]
+    L.spoa_batch_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchStats)]
+    L.spoa_batch_consensus.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_uint32]
+    L.spoa_batch_msa.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_uint64,
+                                 _u32p, _u32p]
+    L.spoa_batch_reversed.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_uint32]
+    L.spoa_synth_new.restype = ctypes.c_void_p
+    L.spoa_synth_new.argtypes = [ctypes.POINTER(SynthSpec), ctypes.c_uint64, ctypes.c_uint32]
+    L.spoa_synth_free.argtypes = [ctypes.c_void_p]
+    L.spoa_synth_sizes.argtypes = [ctypes.c_void_p, _u32p, _u64p, _u64p, _u64p]
+    L.spoa_synth_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
+                                  ctypes.c_void_p, ctypes.c_char_p]
+    # C ABI of the kernels (include/spoa_hip.h), for tuning and budgets
+    L.spoa_hip_last_error.restype = ctypes.c_char_p
+    L.spoa_hip_set_config.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
+                                      ctypes.c_int32, ctypes.c_int32]
+    L.spoa_hip_config.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32,
+                                  _i32p, _i32p]
+    L.spoa_hip_set_memory_budget.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    L.spoa_hip_job_footprint.restype = ctypes.c_uint64
+    L.spoa_hip_job_footprint.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]
+    _lib = L
+    return L
+
+
+def _error():
+    return lib().spoa_last_error().decode()
+
+
+def _b(s):
+    return s.encode() if isinstance(s, str) else bytes(s)
+
+
+class Graph:
+    """spoa::Graph (reference include/spoa/graph.hpp:26-318)."""
+
+    def __init__(self):
+        self._h = lib().spoa_graph_new()
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.spoa_graph_free(self._h)
+            self._h = None
+
+    @property
+    def num_nodes(self):
+        return lib().spoa_graph_num_nodes(self._h)
+
+    def clear(self):
+        lib().spoa_graph_clear(self._h)
+
+    def add_alignment(self, alignment, sequence, weights=1):
+        """Graph::AddAlignment: weights is an int (every base), a list of
+        per-base weights, or a Phred+33 quality string (graph.hpp:134-162)."""
+        s = _b(sequence)
+        flat = (ctypes.c_int32 * max(1, 2 * len(alignment)))()
+        for k, (a, c) in enumerate(alignment):
+            flat[2 * k] = a
+            flat[2 * k + 1] = c
+        if isinstance(weights, (str, bytes)):
+            w = [x - 33 for x in _b(weights)]
+        elif isinstance(weights, int):
+            w = [weights] * len(s)
+        else:
+            w = list(weights)
+        wa = (ctypes.c_uint32 * max(1, len(w)))(*w)
+        if lib().spoa_graph_add_alignment(self._h, flat, len(alignment), s, len(s), wa, len(w)) != 0:
+            raise ValueError(_error())
+
+    def generate_consensus(self):
+        cap = 4 * self.num_nodes + 16
+        buf = ctypes.create_string_buffer(cap)
+        n = lib().spoa_graph_consensus(self._h, buf, cap)
+        return buf.raw[:n].decode()
+
+    def generate_multiple_sequence_alignment(self, include_consensus=False):
+        rows, cols = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        cap = 1 << 16
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            rc = lib().spoa_graph_msa(self._h, int(include_consensus), buf, cap,
+                                      ctypes.byref(rows), ctypes.byref(cols))
+            if rc == -2:
+                cap = rows.value * cols.value + 1
+                continue
+            break
+        raw = buf.raw
+        c = cols.value
+        return [raw[k * c:(k + 1) * c].decode() for k in range(rows.value)]
+
+    def rank_to_node(self):
+        v = self.num_nodes
+        buf = (ctypes.c_uint32 * max(1, v))()
+        n = lib().spoa_graph_ranks(self._h, buf, v)
+        return list(buf[:n])
+
+
+class AlignmentEngine:
+    """spoa::AlignmentEngine (reference include/spoa/alignment_engine.hpp:32-96)
+    whose Align runs on the GPU."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.last_width = 0
+
+    @staticmethod
+    def create(type, m, n, g, e=None, q=None, c=None):
+        if e is None:
+            e = g
+        if q is None:
+            q, c = g, e
+        h = lib().spoa_engine_new(type, m, n, g, e, q, c)
+        if not h:
+            raise ValueError(_error())
+        return AlignmentEngine(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.spoa_engine_free(self._h)
+            self._h = None
+
+    @property
+    def subtype(self):
+        return lib().spoa_engine_subtype(self._h)
+
+    @property
+    def device(self):
+        return lib().spoa_engine_device(self._h)
+
+    def prealloc(self, max_sequence_len, alphabet_size):
+        if lib().spoa_engine_prealloc(self._h, max_sequence_len, alphabet_size) != 0:
+            raise ValueError(_error())
+
+    def align(self, sequence, graph):
+        """Returns (alignment, score); score is None where the reference
+        leaves *score unwritten (empty graph or read, no end cell)."""
+        s = _b(sequence)
+        cap = graph.num_nodes + len(s) + 8
+        buf = (ctypes.c_int32 * (2 * cap))()
+        score = ctypes.c_int32(0)
+        has = ctypes.c_int(0)
+        width = ctypes.c_int(0)
+        n = lib().spoa_align(self._h, graph._h, s, len(s), buf, cap, ctypes.byref(score),
+                             ctypes.byref(has), ctypes.byref(width))
+        if n < 0:
+            raise ValueError(_error())
+        self.last_width = width.value
+        return [(buf[2 * k], buf[2 * k + 1]) for k in range(n)], (score.value if has.value else None)
+
+    def stats(self):
+        st = HipStats()
+        lib().spoa_engine_stats(self._h, ctypes.byref(st))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+
+# ------------------------------------------------------------- windows --
+
+class Windows:
+    """A flattened set of windows: reads of window w are
+    ``seqs[read_off[k]:read_off[k+1]]`` for k in
+    ``range(win_read_off[w], win_read_off[w+1])``."""
+
+    def __init__(self, win_read_off, read_off, seqs, quals=None, templates=None):
+        self.win_read_off = np.ascontiguousarray(win_read_off, dtype=np.uint32)
+        self.read_off = np.ascontiguousarray(read_off, dtype=np.uint64)
+        self.seqs = bytes(seqs)
+        self.quals = None if quals is None else bytes(quals)
+        self.templates = templates
+
+    @staticmethod
+    def from_lists(windows, qualities=None):
+        wo, ro, parts, qparts = [0], [0], [], []
+        for w, reads in enumerate(windows):
+            for k, r in enumerate(reads):
+                b = _b(r)
+                parts.append(b)
+                ro.append(ro[-1] + len(b))
+                if qualities is not None:
+                    qparts.append(_b(qualities[w][k]))
+            wo.append(len(ro) - 1)
+        return Windows(wo, ro, b"".join(parts), b"".join(qparts) if qualities is not None else None)
+
+    @property
+    def n_windows(self):
+        return len(self.win_read_off) - 1
+
+    def reads(self, w):
+        lo, hi = int(self.win_read_off[w]), int(self.win_read_off[w + 1])
+        return [self.seqs[int(self.read_off[k]):int(self.read_off[k + 1])].decode() for k in range(lo, hi)]
+
+    def qualities(self, w):
+        if self.quals is None:
+            return None
+        lo, hi = int(self.win_read_off[w]), int(self.win_read_off[w + 1])
+        return [self.quals[int(self.read_off[k]):int(self.read_off[k + 1])].decode() for k in range(lo, hi)]
+
+    def subset(self, idx):
+        return Windows.from_lists([self.reads(w) for w in idx],
+                                  None if self.quals is None else [self.qualities(w) for w in idx])
+
+
+class WindowResults:
+    def __init__(self, consensus, msa, reversed_, stats):
+        self.consensus = consensus
+        self.msa = msa
+        self.reversed = reversed_
+        self.stats = stats
+
+
+def run_windows(windows, type, m, n, g, e=None, q=None, c=None, mode="batched", strand=False,
+                msa=True, threads=0, device=0):
+    """Aligns every read of every window in order into the window's graph,
+    then builds its consensus (and MSA).  mode 'batched': one launch per
+    round over all windows; 'per_call': the drop-in Align, one read at a
+    time."""
+    if e is None:
+        e = g
+    if q is None:
+        q, c = g, e
+    if not isinstance(windows, Windows):
+        windows = Windows.from_lists(windows)
+    o = BatchOpts(type=type, m=m, n=n, g=g, e=e, q=q, c=c, strand=int(strand), msa=int(msa),
+                  threads=threads, device=device)
+    md = 1 if mode == "per_call" else 0
+    h = lib().spoa_batch_run(md, ctypes.byref(o), windows.n_windows,
+                             windows.win_read_off.ctypes.data, windows.read_off.ctypes.data,
+                             windows.seqs, windows.quals)
+    if not h:
+        raise ValueError(_error())
+    try:
+        st = BatchStats()
+        lib().spoa_batch_get_stats(h, ctypes.byref(st))
+        cons, msas, revs = [], [], []
+        for w in range(windows.n_windows):
+            cap = 1 << 16
+            while True:
+                buf = ctypes.create_string_buffer(cap)
+                k = lib().spoa_batch_consensus(h, w, buf, cap)
+                if k == -2:
+                    cap *= 4
+                    continue
+                break
+            cons.append(buf.raw[:k].decode())
+            if msa:
+                rows, cols = ctypes.c_uint32(0), ctypes.c_uint32(0)
+                cap = 1 << 16
+                while True:
+                    buf = ctypes.create_string_buffer(cap)
+                    rc = lib().spoa_batch_msa(h, w, buf, cap, ctypes.byref(rows), ctypes.byref(cols))
+                    if rc == -2:
+                        cap = rows.value * cols.value + 1
+                        continue
+                    break
+                cc = cols.value
+                raw = buf.raw
+                msas.append([raw[i * cc:(i + 1) * cc].decode() for i in range(rows.value)])
+            if strand:
+                nr = int(windows.win_read_off[w + 1] - windows.win_read_off[w])
+                rb = ctypes.create_string_buffer(max(1, nr))
+                k = lib().spoa_batch_reversed(h, w, rb, nr)
+                revs.append(list(rb.raw[:k]))
+        return WindowResults(cons, msas, revs, st.as_dict())
+    finally:
+        lib().spoa_batch_free(h)
+
+
+def synth_windows(seed, n_windows, template=(500, 500), reads=(20, 20), read_len=(0, 0),
+                  err=(0.1, 0.1), revcomp_odd=False, log_uniform_len=False):
+    """Seeded synthetic windows: uniform ACGT templates, reads as template
+    substrings (read as a circle) or whole templates, per-base errors split
+    evenly into substitutions, insertions and deletions."""
+    sp = SynthSpec(template[0], template[1], reads[0], reads[1], read_len[0], read_len[1],
+                   err[0], err[1], int(revcomp_odd), int(log_uniform_len))
+    h = lib().spoa_synth_new(ctypes.byref(sp), seed, n_windows)
+    if not h:
+        raise ValueError(_error())
+    try:
+        nw, nr = ctypes.c_uint32(0), ctypes.c_uint64(0)
+        sb, tb = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        lib().spoa_synth_sizes(h, ctypes.byref(nw), ctypes.byref(nr), ctypes.byref(sb), ctypes.byref(tb))
+        wo = np.zeros(nw.value + 1, dtype=np.uint32)
+        ro = np.zeros(nr.value + 1, dtype=np.uint64)
+        to = np.zeros(nw.value + 1, dtype=np.uint64)
+        seqs = ctypes.create_string_buffer(max(1, sb.value))
+        tpl = ctypes.create_string_buffer(max(1, tb.value))
+        lib().spoa_synth_copy(h, wo.ctypes.data, ro.ctypes.data, seqs, to.ctypes.data, tpl)
+        templates = [tpl.raw[int(to[w]):int(to[w + 1])].decode() for w in range(nw.value)]
+        return Windows(wo, ro, seqs.raw[:sb.value], None, templates)
+    finally:
+        lib().spoa_synth_free(h)
+
+
+# The five workloads (window shapes and scoring) the benchmark and tests use.
+CONFIGS = {
+    1: dict(name="cfg1_nw_linear_500bp_20reads", n_windows=1, template=(500, 500), reads=(20, 20),
+            read_len=(0, 0), err=(0.10, 0.10), params=(NW, 3, -5, -3)),
+    2: dict(name="cfg2_sw_convex_1.5kb_30reads", n_windows=1, template=(1500, 1500), reads=(30, 30),
+            read_len=(1000, 2000), err=(0.15, 0.15), params=(SW, 5, -4, -8, -6, -10, -4)),
+    3: dict(name="cfg3_ov_affine_500bp_4096windows", n_windows=4096, template=(500, 500),
+            reads=(10, 50), read_len=(400, 600), err=(0.12, 0.12), params=(OV, 5, -4, -8, -6)),
+    4: dict(name="cfg4_nw_convex_5-10kb_256windows_strand", n_windows=256, template=(5000, 10000),
+            reads=(8, 32), read_len=(0, 0), err=(0.10, 0.10), revcomp_odd=True, strand=True,
+            params=(NW, 5, -4, -8, -6, -10, -4)),
+}
+
+
+def config_windows(cfg, seed, n_windows=None):
+    c = CONFIGS[cfg]
+    return synth_windows(seed, c["n_windows"] if n_windows is None else n_windows,
+                         template=c["template"], reads=c["reads"], read_len=c["read_len"],
+                         err=c["err"], revcomp_odd=c.get("revcomp_odd", False))

@@ -1,0 +1,312 @@
+// Batched and per-call POA window drivers (see batch.hpp).
+
+#include "batch.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <stdexcept>
+
+#include "spoa_amd_host.hpp"
+
+namespace spoa_amd {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double Ms(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); }
+
+// IUPAC complement, upper-cased (the strand-ambiguous caller of
+// src/main.cpp:285-302 uses biosoup::Sequence::ReverseAndComplement, a
+// dependency the reference does not vendor; this restates the
+// Watson-Crick / IUPAC table).
+char Complement(char b) {
+  switch (b) {
+    case 'A': case 'a': return 'T';
+    case 'C': case 'c': return 'G';
+    case 'G': case 'g': return 'C';
+    case 'T': case 't': return 'A';
+    case 'U': case 'u': return 'A';
+    case 'R': case 'r': return 'Y';
+    case 'Y': case 'y': return 'R';
+    case 'K': case 'k': return 'M';
+    case 'M': case 'm': return 'K';
+    case 'B': case 'b': return 'V';
+    case 'D': case 'd': return 'H';
+    case 'H': case 'h': return 'D';
+    case 'V': case 'v': return 'B';
+    case 'S': case 's': return 'S';
+    case 'W': case 'w': return 'W';
+    default: return b;
+  }
+}
+
+std::vector<std::uint32_t> Weights(const char* qual, std::size_t len) {
+  std::vector<std::uint32_t> w(len);
+  for (std::size_t i = 0; i < len; ++i) w[i] = static_cast<std::uint32_t>(qual[i] - 33);
+  return w;
+}
+
+}  // namespace
+
+std::string ReverseComplement(const char* s, std::size_t len) {
+  std::string out(len, 'N');
+  for (std::size_t i = 0; i < len; ++i) out[len - 1 - i] = Complement(s[i]);
+  return out;
+}
+
+// ------------------------------------------------------------ ThreadPool --
+
+ThreadPool::ThreadPool(unsigned threads) {
+  if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+  for (unsigned t = 1; t < threads; ++t) workers_.emplace_back([this] { Work(); });
+}
+
+ThreadPool::~ThreadPool() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  for (auto& t : workers_) t.join();
+}
+
+void ThreadPool::Work() {
+  std::uint64_t seen = 0;
+  std::unique_lock<std::mutex> lk(mu_);
+  while (true) {
+    cv_.wait(lk, [&] { return stop_ || epoch_ != seen; });
+    if (stop_) return;
+    seen = epoch_;
+    ++active_;
+    // items are taken only while the loop that published them is current
+    while (epoch_ == seen && next_ < n_) {
+      const std::size_t i = next_++;
+      const std::function<void(std::size_t)>* fn = fn_;
+      lk.unlock();
+      (*fn)(i);
+      lk.lock();
+      ++finished_;
+    }
+    --active_;
+    done_cv_.notify_all();
+  }
+}
+
+void ThreadPool::For(std::size_t n, const std::function<void(std::size_t)>& fn) {
+  if (n == 0) return;
+  std::unique_lock<std::mutex> lk(mu_);
+  fn_ = &fn;
+  n_ = n;
+  next_ = 0;
+  finished_ = 0;
+  ++epoch_;
+  cv_.notify_all();
+  while (next_ < n_) {
+    const std::size_t i = next_++;
+    lk.unlock();
+    fn(i);
+    lk.lock();
+    ++finished_;
+  }
+  done_cv_.wait(lk, [&] { return finished_ == n_ && active_ == 0; });
+  fn_ = nullptr;
+}
+
+// --------------------------------------------------------------- drivers --
+
+std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats) {
+  const auto t0 = Clock::now();
+  BatchStats st;
+  auto engine = spoa::AlignmentEngine::Create(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c);
+  auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(engine.get());
+  if (!hip) throw std::invalid_argument("[spoa::hip] the alignment engine is not the HIP engine");
+  spoa_hip_engine* dev = hip->device();
+  ThreadPool pool(opt.threads);
+
+  const std::uint32_t W = ws.n_windows;
+  std::vector<spoa::Graph> graphs(W);
+  std::vector<WindowResult> out(W);
+  std::uint32_t max_reads = 0;
+  for (std::uint32_t w = 0; w < W; ++w) {
+    max_reads = std::max(max_reads, ws.win_read_off[w + 1] - ws.win_read_off[w]);
+  }
+  const int per_read = opt.strand ? 2 : 1;
+  std::vector<std::string> rc_seq(W);
+  std::vector<std::uint32_t> live;
+  std::vector<spoa_hip_job> jobs;
+  std::vector<std::int32_t> job_of(W * static_cast<std::size_t>(per_read), -1);
+  std::vector<std::vector<std::int32_t>> pair_buf(W * static_cast<std::size_t>(per_read));
+
+  for (std::uint32_t r = 0; r < max_reads; ++r) {
+    live.clear();
+    for (std::uint32_t w = 0; w < W; ++w) {
+      if (ws.win_read_off[w] + r < ws.win_read_off[w + 1]) live.push_back(w);
+    }
+    // build the jobs of this round (reverse complements in parallel)
+    if (opt.strand) {
+      pool.For(live.size(), [&](std::size_t x) {
+        const std::uint32_t w = live[x];
+        const std::uint64_t k = ws.win_read_off[w] + r;
+        rc_seq[w] = ReverseComplement(ws.seqs + ws.read_off[k], ws.read_off[k + 1] - ws.read_off[k]);
+      });
+    }
+    jobs.clear();
+    for (std::uint32_t w : live) {
+      const std::uint64_t k = ws.win_read_off[w] + r;
+      const char* s = ws.seqs + ws.read_off[k];
+      const std::uint32_t len = static_cast<std::uint32_t>(ws.read_off[k + 1] - ws.read_off[k]);
+      for (int d = 0; d < per_read; ++d) {
+        const std::size_t slot = static_cast<std::size_t>(w) * per_read + d;
+        job_of[slot] = -1;
+        if (graphs[w].nodes().empty() || len == 0) continue;
+        spoa_hip_job j{};
+        j.graph = spoa::HipAlignmentEngine::View(graphs[w]);
+        j.seq = d == 0 ? s : rc_seq[w].data();
+        j.len = len;
+        const std::uint32_t cap = j.graph.num_nodes + len;
+        pair_buf[slot].resize(2ull * cap);
+        j.pairs = pair_buf[slot].data();
+        j.cap = cap;
+        job_of[slot] = static_cast<std::int32_t>(jobs.size());
+        jobs.push_back(j);
+      }
+    }
+    if (!jobs.empty()) {
+      const auto ta = Clock::now();
+      const int rc = spoa_hip_align_batch(dev, jobs.data(), static_cast<std::uint32_t>(jobs.size()));
+      st.align_ms += Ms(ta, Clock::now());
+      if (rc != SPOA_HIP_OK) throw std::invalid_argument(spoa_hip_last_error());
+      spoa_hip_stats hs{};
+      spoa_hip_last_stats(dev, &hs);
+      st.kernel_ms += hs.kernel_ms;
+      st.h2d_ms += hs.h2d_ms;
+      st.d2h_ms += hs.d2h_ms;
+      st.cells += hs.cells;
+      st.dp_bytes += hs.dp_bytes;
+      st.launches += hs.launches;
+      st.align_calls += jobs.size();
+    }
+    // merge the alignments (src/main.cpp:275-314 per window)
+    const auto tg = Clock::now();
+    std::mutex err_mu;
+    std::string err;
+    pool.For(live.size(), [&](std::size_t x) {
+      try {
+      const std::uint32_t w = live[x];
+      const std::uint64_t k = ws.win_read_off[w] + r;
+      const char* s = ws.seqs + ws.read_off[k];
+      const std::uint32_t len = static_cast<std::uint32_t>(ws.read_off[k + 1] - ws.read_off[k]);
+      auto alignment_of = [&](int d, std::int32_t* score) {
+        spoa::Alignment a;
+        const std::int32_t ji = job_of[static_cast<std::size_t>(w) * per_read + d];
+        if (ji < 0) return a;
+        const spoa_hip_job& j = jobs[ji];
+        a.resize(j.n_pairs);
+        for (std::uint32_t p = 0; p < j.n_pairs; ++p) a[p] = {j.pairs[2 * p], j.pairs[2 * p + 1]};
+        if (j.has_score) *score = j.score;
+        return a;
+      };
+      std::int32_t score = 0, score_rev = 0;
+      spoa::Alignment a = alignment_of(0, &score);
+      bool reversed = false;
+      if (opt.strand) {
+        spoa::Alignment b = alignment_of(1, &score_rev);
+        if (!(score >= score_rev)) {
+          a.swap(b);
+          reversed = true;
+        }
+        out[w].reversed.push_back(reversed ? 1 : 0);
+      }
+      const char* seq = reversed ? rc_seq[w].data() : s;
+      if (ws.quals) {
+        const char* q = ws.quals + ws.read_off[k];
+        std::vector<std::uint32_t> wt = Weights(q, len);
+        if (reversed) std::reverse(wt.begin(), wt.end());
+        graphs[w].AddAlignment(a, seq, len, wt);
+      } else {
+        graphs[w].AddAlignment(a, seq, len);
+      }
+      } catch (const std::exception& ex) {
+        std::lock_guard<std::mutex> lk(err_mu);
+        if (err.empty()) err = ex.what();
+      }
+    });
+    if (!err.empty()) throw std::invalid_argument(err);
+    st.graph_ms += Ms(tg, Clock::now());
+    ++st.rounds;
+  }
+  const auto tf = Clock::now();
+  pool.For(W, [&](std::size_t w) {
+    out[w].consensus = graphs[w].GenerateConsensus();
+    if (opt.msa) out[w].msa = graphs[w].GenerateMultipleSequenceAlignment();
+  });
+  st.finish_ms = Ms(tf, Clock::now());
+  st.wall_ms = Ms(t0, Clock::now());
+  if (stats) *stats = st;
+  return out;
+}
+
+std::vector<WindowResult> RunPerCall(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats) {
+  const auto t0 = Clock::now();
+  BatchStats st;
+  auto engine = spoa::AlignmentEngine::Create(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c);
+  auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(engine.get());
+  std::vector<WindowResult> out(ws.n_windows);
+  for (std::uint32_t w = 0; w < ws.n_windows; ++w) {
+    spoa::Graph graph;
+    for (std::uint64_t k = ws.win_read_off[w]; k < ws.win_read_off[w + 1]; ++k) {
+      const char* s = ws.seqs + ws.read_off[k];
+      const std::uint32_t len = static_cast<std::uint32_t>(ws.read_off[k + 1] - ws.read_off[k]);
+      std::int32_t score = 0, score_rev = 0;
+      const auto ta = Clock::now();
+      spoa::Alignment a = engine->Align(s, len, graph, &score);
+      auto account = [&] {
+        if (!hip || graph.nodes().empty() || len == 0) return;
+        spoa_hip_stats hs{};
+        spoa_hip_last_stats(hip->device(), &hs);
+        st.kernel_ms += hs.kernel_ms;
+        st.h2d_ms += hs.h2d_ms;
+        st.d2h_ms += hs.d2h_ms;
+        st.cells += hs.cells;
+        st.dp_bytes += hs.dp_bytes;
+        st.launches += hs.launches;
+        st.align_calls += 1;
+      };
+      account();
+      bool reversed = false;
+      std::string rc;
+      if (opt.strand) {
+        rc = ReverseComplement(s, len);
+        spoa::Alignment b = engine->Align(rc.data(), len, graph, &score_rev);
+        account();
+        if (!(score >= score_rev)) {
+          a.swap(b);
+          reversed = true;
+        }
+        out[w].reversed.push_back(reversed ? 1 : 0);
+      }
+      st.align_ms += Ms(ta, Clock::now());
+      const auto tg = Clock::now();
+      const char* seq = reversed ? rc.data() : s;
+      if (ws.quals) {
+        std::vector<std::uint32_t> wt = Weights(ws.quals + ws.read_off[k], len);
+        if (reversed) std::reverse(wt.begin(), wt.end());
+        graph.AddAlignment(a, seq, len, wt);
+      } else {
+        graph.AddAlignment(a, seq, len);
+      }
+      st.graph_ms += Ms(tg, Clock::now());
+    }
+    const auto tf = Clock::now();
+    out[w].consensus = graph.GenerateConsensus();
+    if (opt.msa) out[w].msa = graph.GenerateMultipleSequenceAlignment();
+    st.finish_ms += Ms(tf, Clock::now());
+  }
+  st.wall_ms = Ms(t0, Clock::now());
+  if (stats) *stats = st;
+  return out;
+}
+
+}  // namespace spoa_amd

@@ -1,0 +1,95 @@
+// Batched POA windows on one GPU: many independent windows (each its own
+// graph and reads) advance one read per round; every round aligns the
+// current read of every live window in one persistent-kernel launch, then
+// merges the alignments into the graphs on a host thread pool.
+//
+// Also the plain per-call form (the drop-in AlignmentEngine::Align, one
+// window and one read at a time) over the same inputs, so both forms can
+// be compared on identical bytes.
+#pragma once
+
+#include <condition_variable>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "spoa/spoa.hpp"
+#include "spoa_hip.h"
+
+namespace spoa_amd {
+
+// A fixed pool of host threads running index-parallel loops.
+class ThreadPool {
+ public:
+  explicit ThreadPool(unsigned threads);
+  ~ThreadPool();
+  unsigned size() const { return static_cast<unsigned>(workers_.size()) + 1; }
+  // runs fn(i) for i in [0, n), the calling thread included
+  void For(std::size_t n, const std::function<void(std::size_t)>& fn);
+
+ private:
+  void Work();
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(std::size_t)>* fn_ = nullptr;
+  std::size_t n_ = 0, next_ = 0, finished_ = 0;
+  std::uint64_t epoch_ = 0;
+  unsigned active_ = 0;
+  bool stop_ = false;
+};
+
+// Flattened windows: window w holds reads [win_read_off[w], win_read_off[w+1]);
+// read k is seqs[read_off[k], read_off[k+1]) (quals likewise, or null).
+struct WindowSet {
+  std::uint32_t n_windows = 0;
+  const std::uint32_t* win_read_off = nullptr;
+  const std::uint64_t* read_off = nullptr;
+  const char* seqs = nullptr;
+  const char* quals = nullptr;
+};
+
+struct BatchOptions {
+  spoa::AlignmentType type = spoa::AlignmentType::kNW;
+  std::int8_t m = 5, n = -4, g = -8, e = -6, q = -10, c = -4;
+  bool strand = false;     // align both strands, keep the better (src/main.cpp:285-302)
+  bool msa = true;         // also build the MSA of every window
+  unsigned threads = 0;    // host threads (0: all hardware threads)
+  int device = 0;
+};
+
+struct WindowResult {
+  std::string consensus;
+  std::vector<std::string> msa;
+  std::vector<std::uint8_t> reversed;  // strand mode: read kept reverse-complemented
+};
+
+struct BatchStats {
+  double wall_ms = 0;        // whole run
+  double align_ms = 0;       // inside the alignment calls (pack + copies + kernels + unpack)
+  double kernel_ms = 0;      // alignment kernels (HIP events)
+  double h2d_ms = 0, d2h_ms = 0;
+  double graph_ms = 0;       // AddAlignment + topological sort
+  double finish_ms = 0;      // consensus + MSA
+  std::uint64_t cells = 0;   // sum of V * n over the DPs
+  std::uint64_t dp_bytes = 0;
+  std::uint64_t align_calls = 0;
+  std::uint32_t launches = 0;
+  std::uint32_t rounds = 0;
+};
+
+// Batched form.  Throws std::invalid_argument with the engine's message on
+// failure.
+std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats);
+
+// Plain per-call form: AlignmentEngine::Create(...)->Align for every read
+// of every window, in order, on the calling thread.
+std::vector<WindowResult> RunPerCall(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats);
+
+std::string ReverseComplement(const char* s, std::size_t len);
+
+}  // namespace spoa_amd

@@ -1,0 +1,306 @@
+// C entry points over the C++ drop-in (spoa::AlignmentEngine, spoa::Graph),
+// the batched / per-call window drivers and the synthetic window
+// generator, for callers that bind C (the Python package, ctypes).
+// Declared in include/spoa_capi.h.
+
+#include "spoa_capi.h"
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+#include "spoa/spoa.hpp"
+#include "spoa_amd_host.hpp"
+#include "synth.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+int Error(const std::exception& ex) {
+  g_error = ex.what();
+  return -1;
+}
+
+spoa::AlignmentEngine* Eng(void* p) { return static_cast<spoa::AlignmentEngine*>(p); }
+spoa::Graph* Gr(void* p) { return static_cast<spoa::Graph*>(p); }
+
+struct BatchHandle {
+  std::vector<spoa_amd::WindowResult> results;
+  spoa_amd::BatchStats stats;
+};
+
+struct SynthHandle {
+  std::vector<std::uint32_t> win_read_off;
+  std::vector<std::uint64_t> read_off;
+  std::string seqs;
+  std::string templates;
+  std::vector<std::uint64_t> templ_off;
+};
+
+spoa_amd::BatchOptions Options(const spoa_batch_opts* o) {
+  spoa_amd::BatchOptions b;
+  b.type = static_cast<spoa::AlignmentType>(o->type);
+  b.m = static_cast<std::int8_t>(o->m);
+  b.n = static_cast<std::int8_t>(o->n);
+  b.g = static_cast<std::int8_t>(o->g);
+  b.e = static_cast<std::int8_t>(o->e);
+  b.q = static_cast<std::int8_t>(o->q);
+  b.c = static_cast<std::int8_t>(o->c);
+  b.strand = o->strand != 0;
+  b.msa = o->msa != 0;
+  b.threads = o->threads > 0 ? static_cast<unsigned>(o->threads) : 0u;
+  b.device = o->device;
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spoa_last_error(void) { return g_error.c_str(); }
+
+// ------------------------------------------------------------ engine --
+
+void* spoa_engine_new(int type, int m, int n, int g, int e, int q, int c) {
+  try {
+    return spoa::AlignmentEngine::Create(static_cast<spoa::AlignmentType>(type), static_cast<std::int8_t>(m),
+                                         static_cast<std::int8_t>(n), static_cast<std::int8_t>(g),
+                                         static_cast<std::int8_t>(e), static_cast<std::int8_t>(q),
+                                         static_cast<std::int8_t>(c))
+        .release();
+  } catch (const std::exception& ex) {
+    Error(ex);
+    return nullptr;
+  }
+}
+
+void spoa_engine_free(void* p) { delete Eng(p); }
+
+int spoa_engine_subtype(void* p) { return static_cast<int>(Eng(p)->subtype()); }
+
+int spoa_engine_prealloc(void* p, uint32_t max_len, uint32_t alphabet) {
+  try {
+    Eng(p)->Prealloc(max_len, static_cast<std::uint8_t>(alphabet));
+    return 0;
+  } catch (const std::exception& ex) {
+    return Error(ex);
+  }
+}
+
+int spoa_engine_stats(void* p, spoa_hip_stats* out) {
+  auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(Eng(p));
+  if (!hip || !out) return -1;
+  return spoa_hip_last_stats(hip->device(), out) == SPOA_HIP_OK ? 0 : -1;
+}
+
+spoa_hip_engine* spoa_engine_device(void* p) {
+  auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(Eng(p));
+  return hip ? hip->device() : nullptr;
+}
+
+int spoa_align(void* eng, void* graph, const char* seq, uint32_t len, int32_t* pairs_out, uint32_t cap,
+               int32_t* score, int* has_score, int* width_bits) {
+  try {
+    const std::int32_t kUnset = INT32_MIN;
+    std::int32_t s = kUnset;
+    auto a = Eng(eng)->Align(seq, len, *Gr(graph), &s);
+    if (a.size() > cap) return -2;
+    for (std::size_t k = 0; k < a.size(); ++k) {
+      pairs_out[2 * k] = a[k].first;
+      pairs_out[2 * k + 1] = a[k].second;
+    }
+    if (has_score) *has_score = s != kUnset ? 1 : 0;
+    if (s != kUnset && score) *score = s;
+    if (width_bits) {
+      auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(Eng(eng));
+      *width_bits = hip ? hip->last_width() : 0;
+    }
+    return static_cast<int>(a.size());
+  } catch (const std::exception& ex) {
+    return Error(ex);
+  }
+}
+
+// ------------------------------------------------------------- graph --
+
+void* spoa_graph_new(void) { return new spoa::Graph(); }
+void spoa_graph_free(void* p) { delete Gr(p); }
+void spoa_graph_clear(void* p) { Gr(p)->Clear(); }
+uint32_t spoa_graph_num_nodes(void* p) { return static_cast<uint32_t>(Gr(p)->nodes().size()); }
+
+int spoa_graph_add_alignment(void* graph, const int32_t* pairs, uint32_t n_pairs, const char* seq, uint32_t len,
+                             const uint32_t* weights, uint32_t n_weights) {
+  try {
+    spoa::Alignment a(n_pairs);
+    for (uint32_t k = 0; k < n_pairs; ++k) a[k] = {pairs[2 * k], pairs[2 * k + 1]};
+    if (weights) {
+      std::vector<std::uint32_t> w(weights, weights + n_weights);
+      Gr(graph)->AddAlignment(a, seq, len, w);
+    } else {
+      Gr(graph)->AddAlignment(a, seq, len);
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    return Error(ex);
+  }
+}
+
+int spoa_graph_consensus(void* graph, char* out, uint32_t cap) {
+  const std::string s = Gr(graph)->GenerateConsensus();
+  if (s.size() > cap) return -2;
+  std::memcpy(out, s.data(), s.size());
+  return static_cast<int>(s.size());
+}
+
+int spoa_graph_msa(void* graph, int include_consensus, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols) {
+  auto m = Gr(graph)->GenerateMultipleSequenceAlignment(include_consensus != 0);
+  *rows = static_cast<uint32_t>(m.size());
+  *cols = m.empty() ? 0 : static_cast<uint32_t>(m[0].size());
+  const uint64_t need = static_cast<uint64_t>(*rows) * *cols;
+  if (need > cap) return -2;
+  for (std::size_t r = 0; r < m.size(); ++r) std::memcpy(out + r * (*cols), m[r].data(), *cols);
+  return 0;
+}
+
+int spoa_graph_ranks(void* graph, uint32_t* out, uint32_t cap) {
+  const auto& r = Gr(graph)->rank_to_node();
+  if (r.size() > cap) return -2;
+  for (std::size_t k = 0; k < r.size(); ++k) out[k] = r[k]->id;
+  return static_cast<int>(r.size());
+}
+
+// ------------------------------------------------------------- batch --
+
+void* spoa_batch_run(int mode, const spoa_batch_opts* opts, uint32_t n_windows, const uint32_t* win_read_off,
+                     const uint64_t* read_off, const char* seqs, const char* quals) {
+  try {
+    spoa_amd::WindowSet ws;
+    ws.n_windows = n_windows;
+    ws.win_read_off = win_read_off;
+    ws.read_off = read_off;
+    ws.seqs = seqs;
+    ws.quals = quals;
+    auto h = std::make_unique<BatchHandle>();
+    const spoa_amd::BatchOptions o = Options(opts);
+    if (mode == SPOA_BATCH_PER_CALL) {
+      h->results = spoa_amd::RunPerCall(ws, o, &h->stats);
+    } else {
+      h->results = spoa_amd::RunBatched(ws, o, &h->stats);
+    }
+    return h.release();
+  } catch (const std::exception& ex) {
+    Error(ex);
+    return nullptr;
+  }
+}
+
+void spoa_batch_free(void* h) { delete static_cast<BatchHandle*>(h); }
+
+int spoa_batch_get_stats(void* h, spoa_batch_stats* out) {
+  const auto& s = static_cast<BatchHandle*>(h)->stats;
+  out->wall_ms = s.wall_ms;
+  out->align_ms = s.align_ms;
+  out->kernel_ms = s.kernel_ms;
+  out->h2d_ms = s.h2d_ms;
+  out->d2h_ms = s.d2h_ms;
+  out->graph_ms = s.graph_ms;
+  out->finish_ms = s.finish_ms;
+  out->cells = s.cells;
+  out->dp_bytes = s.dp_bytes;
+  out->align_calls = s.align_calls;
+  out->launches = s.launches;
+  out->rounds = s.rounds;
+  return 0;
+}
+
+int spoa_batch_consensus(void* h, uint32_t w, char* out, uint32_t cap) {
+  const auto& r = static_cast<BatchHandle*>(h)->results;
+  if (w >= r.size()) return -1;
+  const std::string& s = r[w].consensus;
+  if (s.size() > cap) return -2;
+  std::memcpy(out, s.data(), s.size());
+  return static_cast<int>(s.size());
+}
+
+int spoa_batch_msa(void* h, uint32_t w, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols) {
+  const auto& r = static_cast<BatchHandle*>(h)->results;
+  if (w >= r.size()) return -1;
+  const auto& m = r[w].msa;
+  *rows = static_cast<uint32_t>(m.size());
+  *cols = m.empty() ? 0 : static_cast<uint32_t>(m[0].size());
+  const uint64_t need = static_cast<uint64_t>(*rows) * *cols;
+  if (need > cap) return -2;
+  for (std::size_t k = 0; k < m.size(); ++k) std::memcpy(out + k * (*cols), m[k].data(), *cols);
+  return 0;
+}
+
+int spoa_batch_reversed(void* h, uint32_t w, uint8_t* out, uint32_t cap) {
+  const auto& r = static_cast<BatchHandle*>(h)->results;
+  if (w >= r.size()) return -1;
+  const auto& v = r[w].reversed;
+  if (v.size() > cap) return -2;
+  std::memcpy(out, v.data(), v.size());
+  return static_cast<int>(v.size());
+}
+
+// ------------------------------------------------------------- synth --
+
+void* spoa_synth_new(const spoa_synth_spec* spec, uint64_t seed, uint32_t n_windows) {
+  try {
+    spoa_synth::WindowSpec sp;
+    sp.template_min = spec->template_min;
+    sp.template_max = spec->template_max;
+    sp.reads_min = spec->reads_min;
+    sp.reads_max = spec->reads_max;
+    sp.read_len_min = spec->read_len_min;
+    sp.read_len_max = spec->read_len_max;
+    sp.err_min = spec->err_min;
+    sp.err_max = spec->err_max;
+    sp.revcomp_odd = spec->revcomp_odd;
+    sp.log_uniform_len = spec->log_uniform_len;
+    auto h = std::make_unique<SynthHandle>();
+    h->win_read_off.push_back(0);
+    h->read_off.push_back(0);
+    h->templ_off.push_back(0);
+    for (uint32_t w = 0; w < n_windows; ++w) {
+      const spoa_synth::Window win = spoa_synth::MakeWindow(spoa_synth::WindowSeed(seed, w), sp);
+      for (const auto& r : win.reads) {
+        h->seqs += r;
+        h->read_off.push_back(h->seqs.size());
+      }
+      h->win_read_off.push_back(static_cast<uint32_t>(h->read_off.size() - 1));
+      h->templates += win.templ;
+      h->templ_off.push_back(h->templates.size());
+    }
+    return h.release();
+  } catch (const std::exception& ex) {
+    Error(ex);
+    return nullptr;
+  }
+}
+
+void spoa_synth_free(void* h) { delete static_cast<SynthHandle*>(h); }
+
+void spoa_synth_sizes(void* h, uint32_t* n_windows, uint64_t* n_reads, uint64_t* seq_bytes, uint64_t* templ_bytes) {
+  const auto* s = static_cast<SynthHandle*>(h);
+  *n_windows = static_cast<uint32_t>(s->win_read_off.size() - 1);
+  *n_reads = s->read_off.size() - 1;
+  *seq_bytes = s->seqs.size();
+  *templ_bytes = s->templates.size();
+}
+
+void spoa_synth_copy(void* h, uint32_t* win_read_off, uint64_t* read_off, char* seqs, uint64_t* templ_off,
+                     char* templates) {
+  const auto* s = static_cast<SynthHandle*>(h);
+  std::memcpy(win_read_off, s->win_read_off.data(), s->win_read_off.size() * sizeof(uint32_t));
+  std::memcpy(read_off, s->read_off.data(), s->read_off.size() * sizeof(uint64_t));
+  std::memcpy(seqs, s->seqs.data(), s->seqs.size());
+  if (templ_off) std::memcpy(templ_off, s->templ_off.data(), s->templ_off.size() * sizeof(uint64_t));
+  if (templates) std::memcpy(templates, s->templates.data(), s->templates.size());
+}
+
+}  // extern "C"

@@ -1,0 +1,710 @@
+// Host half of the C ABI in include/spoa_hip.h: device buffers, job
+// packing, launch configuration and the per-call / batched launches of the
+// alignment kernels in poa_kernels.hpp.
+//
+// Per call (AlignmentEngine::Align) one job is packed into pinned memory
+// (rank-ordered CSR, first columns, read), copied with one transfer,
+// aligned by one workgroup of up to 16 waves, and its traceback pairs come
+// back with one transfer.  A batch is grouped by launch configuration,
+// ordered by V * n (largest first, the queue order of the persistent
+// grid) and split into launches whose DP matrices fit the memory budget.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "poa_kernels.hpp"
+#include "spoa_hip.h"
+
+using spoa_hip_dev::DevJob;
+using spoa_hip_dev::KArgs;
+using spoa_hip_dev::KParams;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int Fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+bool HipOk(hipError_t rc, const char* what) {
+  if (rc == hipSuccess) return true;
+  g_last_error = std::string("[spoa::hip] ") + what + ": " + hipGetErrorString(rc);
+  return false;
+}
+
+const char kMsgTooLarge[] = "[spoa::SimdAlignmentEngine::Align] error: too large sequence!";
+const char kMsgOverflow[] = "[spoa::SimdAlignmentEngine::Align] error: possible overflow!";
+const char kMsgNoMemory[] = "[spoa::SimdAlignmentEngine::Align] error: insufficient memory!";
+const char kMsgPreTooLarge[] = "[spoa::SimdAlignmentEngine::Prealloc] error: too large sequence!";
+const char kMsgPreNoMemory[] = "[spoa::SimdAlignmentEngine::Prealloc] error: insufficient memory!";
+
+constexpr uint64_t kAlign = 256;
+uint64_t RoundUp(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+// reference src/alignment_engine.cpp:101-110
+int64_t WorstCase(const spoa_hip_params& p, int64_t i, int64_t j) {
+  auto gap = [&](int64_t len) -> int64_t {
+    return len == 0 ? 0 : std::min<int64_t>(p.g + (len - 1) * p.e, p.q + (len - 1) * p.c);
+  };
+  const int64_t d = i > j ? i - j : j - i;
+  return std::min<int64_t>(-1 * (p.m * std::min(i, j) + gap(d)), gap(i) + gap(j));
+}
+
+// Score width for one call (reference simd...:719-774): 0 means "possible
+// overflow!".
+int WidthBits(const spoa_hip_params& p, uint32_t len, uint32_t V) {
+  const int64_t w = WorstCase(p, static_cast<int64_t>(len) + 8, V);
+  if (w < std::numeric_limits<int32_t>::min() + 1024) return 0;
+  if (w < std::numeric_limits<int16_t>::min() + 1024) return 32;
+  return 16;
+}
+
+int Lanes(int width_bits) { return width_bits == 16 ? 16 : 8; }
+int Matrices(int subtype) { return subtype == 0 ? 1 : (subtype == 1 ? 3 : 5); }
+int ReadMatrices(int subtype) { return subtype == 0 ? 1 : (subtype == 1 ? 2 : 3); }
+
+struct Cfg {
+  int k = 8;
+  int waves = 1;
+  bool operator<(const Cfg& o) const { return std::tie(k, waves) < std::tie(o.k, o.waves); }
+};
+
+// Device buffer, grow-only.
+struct DevBuf {
+  void* p = nullptr;
+  uint64_t size = 0;
+  bool Reserve(uint64_t need, const char* what) {
+    if (need <= size) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    size = 0;
+    const uint64_t want = std::max<uint64_t>(need, need + need / 4);
+    if (hipMalloc(&p, want) == hipSuccess) {
+      size = want;
+      return true;
+    }
+    (void)hipGetLastError();
+    if (hipMalloc(&p, need) == hipSuccess) {
+      size = need;
+      return true;
+    }
+    (void)hipGetLastError();
+    p = nullptr;
+    g_last_error = what;
+    return false;
+  }
+  void Release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    size = 0;
+  }
+};
+
+// Pinned host buffer, grow-only.
+struct PinBuf {
+  void* p = nullptr;
+  uint64_t size = 0;
+  bool Reserve(uint64_t need) {
+    if (need <= size) return true;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    size = 0;
+    const uint64_t want = std::max<uint64_t>(need, need + need / 4);
+    if (!HipOk(hipHostMalloc(&p, want, hipHostMallocDefault), "hipHostMalloc")) return false;
+    size = want;
+    return true;
+  }
+  void Release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    size = 0;
+  }
+  template <typename T>
+  T* At(uint64_t off) {
+    return reinterpret_cast<T*>(static_cast<uint8_t*>(p) + off);
+  }
+};
+
+using KernelFn = void (*)(KArgs);
+
+template <int SUB, typename ST, int K>
+KernelFn PickMaxT(int waves) {
+  if (waves == 1) return spoa_hip_dev::PoaAlignKernel<SUB, ST, K, 64>;
+  return spoa_hip_dev::PoaAlignKernel<SUB, ST, K, 1024>;
+}
+
+template <int SUB>
+KernelFn PickSub(int width_bits, int k, int waves) {
+  if (width_bits == 16) {
+    if constexpr (SUB == 2) {
+      return PickMaxT<SUB, int16_t, 16>(waves);
+    } else {
+      return k == 8 ? PickMaxT<SUB, int16_t, 8>(waves) : PickMaxT<SUB, int16_t, 16>(waves);
+    }
+  }
+  return k == 8 ? PickMaxT<SUB, int32_t, 8>(waves) : PickMaxT<SUB, int32_t, 16>(waves);
+}
+
+KernelFn PickKernel(int subtype, int width_bits, int k, int waves) {
+  switch (subtype) {
+    case 0: return PickSub<0>(width_bits, k, waves);
+    case 1: return PickSub<1>(width_bits, k, waves);
+    default: return PickSub<2>(width_bits, k, waves);
+  }
+}
+
+// max-plus 2x2 product
+void MpMul(const int32_t* a, const int32_t* b, int32_t* out) {
+  int32_t r[4];
+  r[0] = std::max(a[0] + b[0], a[1] + b[2]);
+  r[1] = std::max(a[0] + b[1], a[1] + b[3]);
+  r[2] = std::max(a[2] + b[0], a[3] + b[2]);
+  r[3] = std::max(a[2] + b[1], a[3] + b[3]);
+  std::memcpy(out, r, sizeof(r));
+}
+
+KParams MakeParams(const spoa_hip_params& p, int width_bits, int k) {
+  KParams kp{};
+  kp.type = p.type;
+  kp.m = p.m;
+  kp.mis = p.n;
+  kp.g = p.g;
+  kp.e = p.e;
+  kp.q = p.q;
+  kp.c = p.c;
+  // reference simd...:526-528
+  kp.pad = -1 * std::max(std::max(std::abs(static_cast<int>(p.m)), std::abs(static_cast<int>(p.n))),
+                         std::max(std::abs(static_cast<int>(p.g)), std::abs(static_cast<int>(p.q))));
+  kp.ninf = width_bits == 16 ? std::numeric_limits<int16_t>::min() + 1024
+                             : std::numeric_limits<int32_t>::min() + 1024;
+  kp.decay = p.subtype == 0 ? k * p.g : k * p.e;
+  if (p.subtype == 2) {
+    // one L-column segment: (E, Q) entering the segment -> entering the next
+    const int L = Lanes(width_bits);
+    const int32_t seg[4] = {(L - 1) * p.e + std::max<int32_t>(p.g, p.e), (L - 1) * p.c + p.g,
+                            (L - 1) * p.e + p.q, (L - 1) * p.c + std::max<int32_t>(p.q, p.c)};
+    int32_t lane[4];
+    std::memcpy(lane, seg, sizeof(seg));
+    for (int s = 1; s < k / L; ++s) MpMul(seg, lane, lane);
+    std::memcpy(kp.mpow[0], lane, sizeof(lane));
+    for (int t = 1; t < 7; ++t) MpMul(kp.mpow[t - 1], kp.mpow[t - 1], kp.mpow[t]);
+  }
+  return kp;
+}
+
+// First columns (reference simd...:568-577, 590-599, 607-694): fcF / fcO
+// are the vertical-gap columns, fcH the primary column per alignment type.
+void FirstColumns(const spoa_hip_params& p, const spoa_hip_graph& gr, int32_t* fcH, int32_t* fcF, int32_t* fcO) {
+  const uint32_t V = gr.num_nodes;
+  const int32_t kNinf = std::numeric_limits<int32_t>::min() + 1024;
+  fcF[0] = 0;
+  fcO[0] = 0;
+  for (uint32_t i = 1; i <= V; ++i) {
+    const uint32_t b = gr.pred_off[i - 1], e = gr.pred_off[i];
+    int32_t pf = b == e ? p.g - p.e : kNinf;
+    int32_t po = b == e ? p.q - p.c : kNinf;
+    for (uint32_t k = b; k < e; ++k) {
+      const uint32_t row = gr.pred_rank[k] + 1;
+      pf = std::max(pf, fcF[row]);
+      po = std::max(po, fcO[row]);
+    }
+    fcF[i] = pf + p.e;
+    fcO[i] = po + p.c;
+  }
+  if (p.type == SPOA_HIP_NW) {
+    if (p.subtype == 2) {
+      for (uint32_t i = 0; i <= V; ++i) fcH[i] = std::max(fcF[i], fcO[i]);
+    } else if (p.subtype == 1) {
+      for (uint32_t i = 0; i <= V; ++i) fcH[i] = fcF[i];
+    } else {
+      fcH[0] = 0;
+      for (uint32_t i = 1; i <= V; ++i) {
+        const uint32_t b = gr.pred_off[i - 1], e = gr.pred_off[i];
+        int32_t ph = b == e ? 0 : kNinf;
+        for (uint32_t k = b; k < e; ++k) ph = std::max(ph, fcH[gr.pred_rank[k] + 1]);
+        fcH[i] = ph + p.g;
+      }
+    }
+  } else {
+    for (uint32_t i = 0; i <= V; ++i) fcH[i] = 0;
+  }
+}
+
+}  // namespace
+
+struct spoa_hip_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  spoa_hip_params prm{};
+  int num_cus = 256;
+  uint64_t budget = 0;
+  DevBuf d_in, d_out, d_arena, d_counter;
+  PinBuf h_in, h_out;
+  spoa_hip_stats stats{};
+  std::map<std::pair<KernelFn, int>, int> occupancy;
+  // overrides: (width, batch) -> max_width -> cfg
+  std::map<std::pair<int, int>, std::map<uint32_t, Cfg>> overrides;
+};
+
+namespace {
+
+Cfg DefaultCfg(const spoa_hip_engine* E, uint32_t W, int width_bits, bool batch) {
+  Cfg c;
+  const bool convex16 = E->prm.subtype == 2 && width_bits == 16;
+  if (batch) {
+    c.k = (convex16 || W > 512) ? 16 : 8;
+  } else {
+    c.k = convex16 ? 16 : 8;
+  }
+  const uint32_t per_wave = 64u * static_cast<uint32_t>(c.k);
+  c.waves = static_cast<int>(std::min<uint32_t>(16u, std::max<uint32_t>(1u, (W + per_wave - 1) / per_wave)));
+  return c;
+}
+
+Cfg ChooseCfg(const spoa_hip_engine* E, uint32_t len, int width_bits, bool batch) {
+  const int L = Lanes(width_bits);
+  const uint32_t W = (len + L - 1) / L * L;
+  auto it = E->overrides.find({width_bits, batch ? 1 : 0});
+  if (it != E->overrides.end()) {
+    auto jt = it->second.lower_bound(W);
+    if (jt != it->second.end()) return jt->second;
+  }
+  return DefaultCfg(E, W, width_bits, batch);
+}
+
+struct Plan {
+  uint32_t job;
+  int width;
+  Cfg cfg;
+  uint32_t W, stride;
+  uint64_t mat_bytes;
+  uint64_t graph_words;
+  uint64_t seq_bytes;
+};
+
+uint64_t MatBytes(const spoa_hip_params& p, uint32_t V, uint32_t stride, int width) {
+  return RoundUp(static_cast<uint64_t>(V + 1) * Matrices(p.subtype) * stride * (width / 8), kAlign);
+}
+
+// Packs, launches and unpacks one chunk of jobs that share a configuration.
+int RunChunk(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& plans, size_t lo, size_t hi) {
+  const spoa_hip_params& P = E->prm;
+  const int width = plans[lo].width;
+  const Cfg cfg = plans[lo].cfg;
+  const uint32_t nj = static_cast<uint32_t>(hi - lo);
+
+  // layout of the input block: jobs | graphs | sequences
+  uint64_t graph_words = 0, seq_bytes = 0, mat_elems = 0, pairs = 0;
+  for (size_t x = lo; x < hi; ++x) {
+    graph_words += plans[x].graph_words;
+    seq_bytes += plans[x].seq_bytes;
+    pairs += jobs[plans[x].job].graph.num_nodes + jobs[plans[x].job].len;
+  }
+  const uint64_t off_jobs = 0;
+  const uint64_t off_graph = RoundUp(nj * sizeof(DevJob), kAlign);
+  const uint64_t off_seq = RoundUp(off_graph + graph_words * 4, kAlign);
+  const uint64_t in_bytes = RoundUp(off_seq + seq_bytes, kAlign);
+  const uint64_t off_res = 0;
+  const uint64_t off_pairs = RoundUp(static_cast<uint64_t>(nj) * 16, kAlign);
+  const uint64_t out_bytes = off_pairs + pairs * 8;
+
+  if (!E->h_in.Reserve(in_bytes) || !E->h_out.Reserve(out_bytes)) return SPOA_HIP_DEVICE_ERROR;
+  if (!E->d_in.Reserve(in_bytes, kMsgNoMemory) || !E->d_out.Reserve(out_bytes, kMsgNoMemory)) {
+    return SPOA_HIP_NO_MEMORY;
+  }
+
+  const auto t_host0 = std::chrono::steady_clock::now();
+  DevJob* dj = E->h_in.At<DevJob>(off_jobs);
+  uint64_t gw = 0, sb = 0, pr = 0, me = 0;
+  for (size_t x = lo; x < hi; ++x) {
+    const Plan& pl = plans[x];
+    const spoa_hip_job& j = jobs[pl.job];
+    const spoa_hip_graph& gr = j.graph;
+    const uint32_t V = gr.num_nodes;
+    DevJob& d = dj[x - lo];
+    d.mat_off = me / (width / 8);
+    d.graph_off = static_cast<uint32_t>(gw);
+    d.seq_off = static_cast<uint32_t>(sb);
+    d.V = V;
+    d.n = j.len;
+    d.W = pl.W;
+    d.stride = pl.stride;
+    d.pair_off = static_cast<uint32_t>(pr);
+    d.cap = V + j.len;
+    uint32_t* g = E->h_in.At<uint32_t>(off_graph) + gw;
+    int32_t* node_id = reinterpret_cast<int32_t*>(g);
+    uint32_t* info = g + V;
+    uint32_t* poff = g + 2 * V;
+    int32_t* fcH = reinterpret_cast<int32_t*>(g + 3 * V + 1);
+    int32_t* fcF = fcH + (V + 1);
+    int32_t* fcO = fcF + (V + 1);
+    uint32_t* preds = g + 6 * V + 4;
+    const uint32_t base = gr.pred_off[0];
+    for (uint32_t r = 0; r < V; ++r) {
+      node_id[r] = gr.node_id[r];
+      info[r] = static_cast<uint32_t>(gr.row_char[r]) | (gr.is_sink[r] ? 0x100u : 0u);
+      poff[r] = gr.pred_off[r] - base;
+    }
+    poff[V] = gr.pred_off[V] - base;
+    for (uint32_t k = base; k < gr.pred_off[V]; ++k) preds[k - base] = gr.pred_rank[k] + 1;
+    FirstColumns(P, gr, fcH, fcF, fcO);
+    uint8_t* s = E->h_in.At<uint8_t>(off_seq) + sb;
+    std::memcpy(s, j.seq, j.len);
+    std::memset(s + j.len, 0, pl.seq_bytes - j.len);
+    gw += pl.graph_words;
+    sb += pl.seq_bytes;
+    pr += d.cap;
+    me += pl.mat_bytes;
+  }
+  mat_elems = me;
+  if (!E->d_arena.Reserve(mat_elems, kMsgNoMemory)) return SPOA_HIP_NO_MEMORY;
+  if (!E->d_counter.Reserve(256, kMsgNoMemory)) return SPOA_HIP_NO_MEMORY;
+  const auto t_host1 = std::chrono::steady_clock::now();
+  E->stats.host_ms += std::chrono::duration<double, std::milli>(t_host1 - t_host0).count();
+
+  KernelFn fn = PickKernel(P.subtype, width, cfg.k, cfg.waves);
+  const int threads = 64 * cfg.waves;
+  int per_cu = 0;
+  auto key = std::make_pair(fn, threads);
+  auto oc = E->occupancy.find(key);
+  if (oc == E->occupancy.end()) {
+    if (!HipOk(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0), "occupancy")) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+    E->occupancy[key] = per_cu;
+  } else {
+    per_cu = oc->second;
+  }
+  const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(nj, static_cast<uint32_t>(std::max(1, per_cu)) *
+                                                                          static_cast<uint32_t>(E->num_cus)));
+
+  KArgs args{};
+  args.prm = MakeParams(P, width, cfg.k);
+  uint8_t* din = static_cast<uint8_t*>(E->d_in.p);
+  uint8_t* dout = static_cast<uint8_t*>(E->d_out.p);
+  args.jobs = reinterpret_cast<const DevJob*>(din + off_jobs);
+  args.n_jobs = nj;
+  args.queue = static_cast<uint32_t*>(E->d_counter.p);
+  args.graphs = reinterpret_cast<const uint32_t*>(din + off_graph);
+  args.seqs = din + off_seq;
+  args.arena = E->d_arena.p;
+  args.pairs = reinterpret_cast<int32_t*>(dout + off_pairs);
+  args.results = reinterpret_cast<int32_t*>(dout + off_res);
+
+  hipStream_t st = E->stream;
+  if (!HipOk(hipEventRecord(E->ev0, st), "event") ||
+      !HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in") ||
+      !HipOk(hipEventRecord(E->ev1, st), "event")) {
+    return SPOA_HIP_DEVICE_ERROR;
+  }
+  spoa_hip_dev::ResetCounter<<<1, 64, 0, st>>>(args.queue);
+  fn<<<grid, threads, 0, st>>>(args);
+  if (!HipOk(hipGetLastError(), "kernel launch") || !HipOk(hipEventRecord(E->ev2, st), "event") ||
+      !HipOk(hipMemcpyAsync(E->h_out.p, E->d_out.p, out_bytes, hipMemcpyDeviceToHost, st), "copy out") ||
+      !HipOk(hipEventRecord(E->ev3, st), "event") || !HipOk(hipStreamSynchronize(st), "kernel")) {
+    return SPOA_HIP_DEVICE_ERROR;
+  }
+  float ms_in = 0, ms_k = 0, ms_out = 0;
+  (void)hipEventElapsedTime(&ms_in, E->ev0, E->ev1);
+  (void)hipEventElapsedTime(&ms_k, E->ev1, E->ev2);
+  (void)hipEventElapsedTime(&ms_out, E->ev2, E->ev3);
+  E->stats.h2d_ms += ms_in;
+  E->stats.kernel_ms += ms_k;
+  E->stats.d2h_ms += ms_out;
+  E->stats.launches += 1;
+
+  // unpack: pairs come back in traceback order
+  const int32_t* res = E->h_out.At<int32_t>(off_res);
+  const int32_t* allp = E->h_out.At<int32_t>(off_pairs);
+  int first = SPOA_HIP_OK;
+  for (size_t x = lo; x < hi; ++x) {
+    spoa_hip_job& j = jobs[plans[x].job];
+    const DevJob& d = dj[x - lo];
+    const int32_t* r4 = res + 4 * (x - lo);
+    j.width_bits = width;
+    j.has_score = r4[2];
+    j.score = r4[1];
+    const uint32_t np = static_cast<uint32_t>(r4[0]);
+    int status = r4[3] == spoa_hip_dev::kDevCapacity
+                     ? SPOA_HIP_CAPACITY
+                     : (r4[3] == spoa_hip_dev::kDevInconsistent ? SPOA_HIP_INCONSISTENT : SPOA_HIP_OK);
+    if (status == SPOA_HIP_OK && np > j.cap) status = SPOA_HIP_CAPACITY;
+    if (status == SPOA_HIP_OK) {
+      const int32_t* src = allp + 2ull * d.pair_off;
+      for (uint32_t k = 0; k < np; ++k) {
+        j.pairs[2 * k] = src[2 * (np - 1 - k)];
+        j.pairs[2 * k + 1] = src[2 * (np - 1 - k) + 1];
+      }
+      j.n_pairs = np;
+    } else {
+      j.n_pairs = 0;
+      if (first == SPOA_HIP_OK) {
+        first = status;
+        g_last_error = status == SPOA_HIP_CAPACITY ? "[spoa::hip] alignment output capacity exceeded"
+                                                   : "[spoa::hip] traceback found no consistent move";
+      }
+    }
+    j.status = status;
+  }
+  return first;
+}
+
+int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch) {
+  E->stats = spoa_hip_stats{};
+  const spoa_hip_params& P = E->prm;
+  int first = SPOA_HIP_OK;
+  auto fail = [&](spoa_hip_job& j, int code, const char* msg) {
+    j.status = code;
+    j.n_pairs = 0;
+    j.has_score = 0;
+    if (first == SPOA_HIP_OK) {
+      first = code;
+      g_last_error = msg;
+    }
+  };
+  std::vector<Plan> plans;
+  plans.reserve(n_jobs);
+  for (uint32_t x = 0; x < n_jobs; ++x) {
+    spoa_hip_job& j = jobs[x];
+    j.n_pairs = 0;
+    j.has_score = 0;
+    j.width_bits = 0;
+    j.status = SPOA_HIP_OK;
+    // reference simd...:708-715
+    if (j.len > static_cast<uint32_t>(std::numeric_limits<int32_t>::max())) {
+      fail(j, SPOA_HIP_TOO_LARGE, kMsgTooLarge);
+      continue;
+    }
+    const uint32_t V = j.graph.num_nodes;
+    if (V == 0 || j.len == 0) continue;
+    const int width = WidthBits(P, j.len, V);
+    if (width == 0) {
+      fail(j, SPOA_HIP_OVERFLOW, kMsgOverflow);
+      continue;
+    }
+    Plan pl;
+    pl.job = x;
+    pl.width = width;
+    pl.cfg = ChooseCfg(E, j.len, width, batch);
+    const int L = Lanes(width);
+    pl.W = (j.len + L - 1) / L * L;
+    pl.stride = static_cast<uint32_t>(RoundUp(pl.W, static_cast<uint64_t>(pl.cfg.k)));
+    pl.mat_bytes = MatBytes(P, V, pl.stride, width);
+    const uint32_t E_in = j.graph.pred_off[V] - j.graph.pred_off[0];
+    pl.graph_words = RoundUp(6ull * V + 4 + E_in, 4);
+    pl.seq_bytes = RoundUp(pl.stride + 16, 16);
+    if (pl.mat_bytes > E->budget) {
+      fail(j, SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+      continue;
+    }
+    plans.push_back(pl);
+    // algorithmic work (cells, bytes) of this DP
+    uint64_t R = 0;
+    for (uint32_t r = 0; r < V; ++r) R += std::max<uint32_t>(1u, j.graph.pred_off[r + 1] - j.graph.pred_off[r]);
+    E->stats.cells += static_cast<uint64_t>(V) * j.len;
+    E->stats.dp_bytes += static_cast<uint64_t>(width / 8) * j.len *
+                         (static_cast<uint64_t>(Matrices(P.subtype)) * V + static_cast<uint64_t>(ReadMatrices(P.subtype)) * R);
+    E->stats.jobs += 1;
+  }
+  // group by (width, cfg); within a group the queue runs largest V * n first
+  std::stable_sort(plans.begin(), plans.end(), [&](const Plan& a, const Plan& b) {
+    if (a.width != b.width) return a.width < b.width;
+    if (a.cfg.k != b.cfg.k) return a.cfg.k < b.cfg.k;
+    if (a.cfg.waves != b.cfg.waves) return a.cfg.waves < b.cfg.waves;
+    const uint64_t wa = static_cast<uint64_t>(jobs[a.job].graph.num_nodes) * jobs[a.job].len;
+    const uint64_t wb = static_cast<uint64_t>(jobs[b.job].graph.num_nodes) * jobs[b.job].len;
+    return wa > wb;
+  });
+  size_t lo = 0;
+  while (lo < plans.size()) {
+    size_t hi = lo;
+    uint64_t bytes = 0;
+    while (hi < plans.size() && plans[hi].width == plans[lo].width && plans[hi].cfg.k == plans[lo].cfg.k &&
+           plans[hi].cfg.waves == plans[lo].cfg.waves && bytes + plans[hi].mat_bytes <= E->budget) {
+      bytes += plans[hi].mat_bytes;
+      ++hi;
+    }
+    const int rc = RunChunk(E, jobs, plans, lo, hi);
+    if (rc != SPOA_HIP_OK && first == SPOA_HIP_OK) first = rc;
+    if (rc == SPOA_HIP_DEVICE_ERROR || rc == SPOA_HIP_NO_MEMORY) {
+      const std::string msg = g_last_error;
+      for (size_t x = lo; x < plans.size(); ++x) jobs[plans[x].job].status = rc;
+      g_last_error = msg;
+      return rc;
+    }
+    lo = hi;
+  }
+  return first;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spoa_hip_last_error(void) { return g_last_error.c_str(); }
+
+int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_engine** out) {
+  if (!params || !out) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null argument");
+  if (params->type < 0 || params->type > 2) {
+    return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::AlignmentEngine::Create] error: invalid alignment type!");
+  }
+  if (params->subtype < 0 || params->subtype > 2) {
+    return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] invalid alignment subtype");
+  }
+  int count = 0;
+  if (!HipOk(hipGetDeviceCount(&count), "hipGetDeviceCount")) return SPOA_HIP_DEVICE_ERROR;
+  if (device < 0 || device >= count) return Fail(SPOA_HIP_DEVICE_ERROR, "[spoa::hip] no such HIP device");
+  if (!HipOk(hipSetDevice(device), "hipSetDevice")) return SPOA_HIP_DEVICE_ERROR;
+  auto* E = new spoa_hip_engine();
+  E->device = device;
+  E->prm = *params;
+  hipDeviceProp_t prop;
+  if (HipOk(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties")) E->num_cus = prop.multiProcessorCount;
+  size_t free_b = 0, total_b = 0;
+  if (HipOk(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo")) {
+    E->budget = static_cast<uint64_t>(static_cast<double>(free_b) * 0.80);
+  } else {
+    E->budget = 8ull << 30;
+  }
+  if (!HipOk(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking), "hipStreamCreate") ||
+      !HipOk(hipEventCreate(&E->ev0), "hipEventCreate") || !HipOk(hipEventCreate(&E->ev1), "hipEventCreate") ||
+      !HipOk(hipEventCreate(&E->ev2), "hipEventCreate") || !HipOk(hipEventCreate(&E->ev3), "hipEventCreate")) {
+    spoa_hip_engine_destroy(E);
+    return SPOA_HIP_DEVICE_ERROR;
+  }
+  *out = E;
+  return SPOA_HIP_OK;
+}
+
+int spoa_hip_engine_destroy(spoa_hip_engine* E) {
+  if (!E) return SPOA_HIP_OK;
+  (void)hipSetDevice(E->device);
+  if (E->stream) (void)hipStreamSynchronize(E->stream);
+  E->d_in.Release();
+  E->d_out.Release();
+  E->d_arena.Release();
+  E->d_counter.Release();
+  E->h_in.Release();
+  E->h_out.Release();
+  if (E->ev0) (void)hipEventDestroy(E->ev0);
+  if (E->ev1) (void)hipEventDestroy(E->ev1);
+  if (E->ev2) (void)hipEventDestroy(E->ev2);
+  if (E->ev3) (void)hipEventDestroy(E->ev3);
+  if (E->stream) (void)hipStreamDestroy(E->stream);
+  delete E;
+  return SPOA_HIP_OK;
+}
+
+// reference simd...:381-422
+int spoa_hip_prealloc(spoa_hip_engine* E, uint32_t max_len, uint32_t alphabet_size) {
+  if (!E) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null engine");
+  if (max_len > static_cast<uint32_t>(std::numeric_limits<int32_t>::max())) {
+    return Fail(SPOA_HIP_TOO_LARGE, kMsgPreTooLarge);
+  }
+  const int64_t V = static_cast<int64_t>(max_len) * (alphabet_size & 0xffu);
+  const int64_t w = WorstCase(E->prm, static_cast<int64_t>(max_len) + 8, V);
+  if (w < std::numeric_limits<int32_t>::min() + 1024) return SPOA_HIP_OK;
+  const int width = w < std::numeric_limits<int16_t>::min() + 1024 ? 32 : 16;
+  if (!HipOk(hipSetDevice(E->device), "hipSetDevice")) return SPOA_HIP_DEVICE_ERROR;
+  const Cfg cfg = ChooseCfg(E, max_len, width, false);
+  const int L = Lanes(width);
+  const uint64_t W = (static_cast<uint64_t>(max_len) + L - 1) / L * L;
+  const uint64_t stride = RoundUp(W, static_cast<uint64_t>(cfg.k));
+  const uint64_t bytes =
+      RoundUp(static_cast<uint64_t>(V + 1) * Matrices(E->prm.subtype) * stride * (width / 8), kAlign);
+  if (bytes > E->budget || !E->d_arena.Reserve(bytes, kMsgPreNoMemory)) {
+    return Fail(SPOA_HIP_NO_MEMORY, kMsgPreNoMemory);
+  }
+  return SPOA_HIP_OK;
+}
+
+int spoa_hip_align(spoa_hip_engine* E, const spoa_hip_graph* graph, const char* seq, uint32_t len, int32_t* pairs,
+                   uint32_t cap, uint32_t* n_pairs, int32_t* score, int32_t* has_score) {
+  if (!E || !graph || (!seq && len) || !n_pairs) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null argument");
+  if (!HipOk(hipSetDevice(E->device), "hipSetDevice")) return SPOA_HIP_DEVICE_ERROR;
+  spoa_hip_job j{};
+  j.graph = *graph;
+  j.seq = seq;
+  j.len = len;
+  j.pairs = pairs;
+  j.cap = cap;
+  const int rc = RunBatch(E, &j, 1, false);
+  *n_pairs = j.n_pairs;
+  if (has_score) *has_score = j.has_score;
+  if (score && j.has_score) *score = j.score;
+  return rc;
+}
+
+int spoa_hip_align_batch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs) {
+  if (!E || (!jobs && n_jobs)) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null argument");
+  if (!HipOk(hipSetDevice(E->device), "hipSetDevice")) return SPOA_HIP_DEVICE_ERROR;
+  return RunBatch(E, jobs, n_jobs, true);
+}
+
+int spoa_hip_last_stats(spoa_hip_engine* E, spoa_hip_stats* out) {
+  if (!E || !out) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null argument");
+  *out = E->stats;
+  return SPOA_HIP_OK;
+}
+
+int spoa_hip_config(spoa_hip_engine* E, uint32_t len, int32_t width_bits, int32_t batch, int32_t* k, int32_t* waves) {
+  if (!E || !k || !waves || (width_bits != 16 && width_bits != 32)) {
+    return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] invalid argument");
+  }
+  const Cfg c = ChooseCfg(E, len, width_bits, batch != 0);
+  *k = c.k;
+  *waves = c.waves;
+  return SPOA_HIP_OK;
+}
+
+int spoa_hip_set_config(spoa_hip_engine* E, int32_t width_bits, int32_t batch, uint32_t max_width, int32_t k,
+                        int32_t waves) {
+  if (!E || (width_bits != 16 && width_bits != 32)) {
+    return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] invalid argument");
+  }
+  auto& table = E->overrides[{width_bits, batch ? 1 : 0}];
+  if (k == 0) {
+    table.clear();
+    return SPOA_HIP_OK;
+  }
+  if ((k != 8 && k != 16) || waves < 1 || waves > spoa_hip_dev::kMaxWaves ||
+      (E->prm.subtype == 2 && width_bits == 16 && k != 16)) {
+    return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] unsupported launch configuration");
+  }
+  Cfg c;
+  c.k = k;
+  c.waves = waves;
+  table[max_width] = c;
+  return SPOA_HIP_OK;
+}
+
+int spoa_hip_set_memory_budget(spoa_hip_engine* E, uint64_t bytes) {
+  if (!E) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null engine");
+  E->budget = bytes;
+  return SPOA_HIP_OK;
+}
+
+uint64_t spoa_hip_job_footprint(spoa_hip_engine* E, uint32_t V, uint32_t len, int32_t batch) {
+  if (!E || V == 0 || len == 0) return 0;
+  const int width = WidthBits(E->prm, len, V);
+  if (width == 0) return 0;
+  const Cfg cfg = ChooseCfg(E, len, width, batch != 0);
+  const int L = Lanes(width);
+  const uint32_t W = (len + L - 1) / L * L;
+  const uint32_t stride = static_cast<uint32_t>(RoundUp(W, static_cast<uint64_t>(cfg.k)));
+  return MatBytes(E->prm, V, stride, width);
+}
+
+}  // extern "C"
