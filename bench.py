@@ -113,9 +113,10 @@ def main():
     strand = cfg.get("strand", False)
     windows = spoa_amd.config_windows(args.config, args.seed, n_windows)
 
+    ctx = spoa_amd.BatchContext(*params, strand=strand, msa=True, threads=threads)
+
     def step():
-        return spoa_amd.run_windows(windows, *params, mode="batched", strand=strand, msa=True,
-                                    threads=threads)
+        return ctx.run(windows)
 
     for _ in range(args.warmup):
         step()
@@ -176,7 +177,8 @@ def main():
                      "avg_launch_ms": kernel_ms / max(1, launches),
                      "bytes_per_launch": dp_bytes / max(1, launches)},
         "breakdown_ms_per_step": {k: last[k] for k in ("wall_ms", "align_ms", "kernel_ms", "h2d_ms",
-                                                       "d2h_ms", "graph_ms", "finish_ms")},
+                                                       "d2h_ms", "pack_ms", "alloc_ms", "wait_ms",
+                                                       "unpack_ms", "graph_ms", "finish_ms")},
         "per_call": per_call,
         "batched_vs_per_call": (wps / per_call["windows_per_s"]) if per_call else None,
         "cpu_baseline": cpu,

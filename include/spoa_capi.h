@@ -67,6 +67,7 @@ typedef struct spoa_batch_opts {
 
 typedef struct spoa_batch_stats {
   double wall_ms, align_ms, kernel_ms, h2d_ms, d2h_ms, graph_ms, finish_ms;
+  double pack_ms, alloc_ms, wait_ms, unpack_ms;
   uint64_t cells, dp_bytes, align_calls;
   uint32_t launches, rounds;
 } spoa_batch_stats;
@@ -79,6 +80,14 @@ typedef struct spoa_batch_stats {
 void* spoa_batch_run(int mode, const spoa_batch_opts* opts, uint32_t n_windows, const uint32_t* win_read_off,
                      const uint64_t* read_off, const char* seqs, const char* quals);
 void spoa_batch_free(void* result);
+/* A reusable batched-form context (engine, device buffers, host threads)
+ * for repeated runs with one set of options; runs return results as
+ * spoa_batch_run does. */
+void* spoa_batch_context_new(const spoa_batch_opts* opts);
+void spoa_batch_context_free(void* ctx);
+spoa_hip_engine* spoa_batch_context_device(void* ctx);
+void* spoa_batch_context_run(void* ctx, uint32_t n_windows, const uint32_t* win_read_off, const uint64_t* read_off,
+                             const char* seqs, const char* quals);
 int spoa_batch_get_stats(void* result, spoa_batch_stats* out);
 int spoa_batch_consensus(void* result, uint32_t window, char* out, uint32_t cap);
 int spoa_batch_msa(void* result, uint32_t window, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols);

@@ -110,10 +110,13 @@ int spoa_hip_align_batch(spoa_hip_engine* engine, spoa_hip_job* jobs, uint32_t n
  * S * n * (W * V + Rd * R): every score matrix written once, the
  * predecessor rows read once per in-edge). */
 typedef struct spoa_hip_stats {
-  double kernel_ms;
-  double h2d_ms;
-  double d2h_ms;
-  double host_ms;
+  double kernel_ms;  /* device time of the alignment launches (HIP events) */
+  double h2d_ms;     /* device time of the input copy */
+  double d2h_ms;     /* device time of the output copy */
+  double host_ms;    /* host time packing the jobs */
+  double alloc_ms;   /* host time growing device / pinned buffers */
+  double wait_ms;    /* host time blocked on the stream */
+  double unpack_ms;  /* host time unpacking the pairs */
   uint32_t launches;
   uint32_t jobs;
   uint64_t cells;
@@ -130,7 +133,7 @@ int spoa_hip_config(spoa_hip_engine* engine, uint32_t len, int32_t width_bits, i
 /* Override the launch configuration for rows whose reference width
  * (ceil(len / L) * L) is at most max_width, at one score width and mode;
  * used by the autotuner.  cols_per_lane in {8, 16} (convex int16 needs 16),
- * waves in 1..16.  cols_per_lane == 0 clears all overrides of that width
+ * waves in 1..8.  cols_per_lane == 0 clears all overrides of that width
  * and mode. */
 int spoa_hip_set_config(spoa_hip_engine* engine, int32_t width_bits, int32_t batch, uint32_t max_width,
                         int32_t cols_per_lane, int32_t waves);

@@ -40,6 +40,8 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 class HipStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("h2d_ms", ctypes.c_double),
                 ("d2h_ms", ctypes.c_double), ("host_ms", ctypes.c_double),
+                ("alloc_ms", ctypes.c_double), ("wait_ms", ctypes.c_double),
+                ("unpack_ms", ctypes.c_double),
                 ("launches", ctypes.c_uint32), ("jobs", ctypes.c_uint32),
                 ("cells", ctypes.c_uint64), ("dp_bytes", ctypes.c_uint64)]
 
@@ -53,7 +55,9 @@ class BatchStats(ctypes.Structure):
     _fields_ = [("wall_ms", ctypes.c_double), ("align_ms", ctypes.c_double),
                 ("kernel_ms", ctypes.c_double), ("h2d_ms", ctypes.c_double),
                 ("d2h_ms", ctypes.c_double), ("graph_ms", ctypes.c_double),
-                ("finish_ms", ctypes.c_double), ("cells", ctypes.c_uint64),
+                ("finish_ms", ctypes.c_double), ("pack_ms", ctypes.c_double),
+                ("alloc_ms", ctypes.c_double), ("wait_ms", ctypes.c_double),
+                ("unpack_ms", ctypes.c_double), ("cells", ctypes.c_uint64),
                 ("dp_bytes", ctypes.c_uint64), ("align_calls", ctypes.c_uint64),
                 ("launches", ctypes.c_uint32), ("rounds", ctypes.c_uint32)]
 
@@ -106,6 +110,14 @@ def lib():
     L.spoa_batch_run.argtypes = [ctypes.c_int, ctypes.POINTER(BatchOpts), ctypes.c_uint32,
                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]
     L.spoa_batch_free.argtypes = [ctypes.c_void_p]
+    L.spoa_batch_context_new.restype = ctypes.c_void_p
+    L.spoa_batch_context_new.argtypes = [ctypes.POINTER(BatchOpts)]
+    L.spoa_batch_context_free.argtypes = [ctypes.c_void_p]
+    L.spoa_batch_context_device.restype = ctypes.c_void_p
+    L.spoa_batch_context_device.argtypes = [ctypes.c_void_p]
+    L.spoa_batch_context_run.restype = ctypes.c_void_p
+    L.spoa_batch_context_run.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_char_p, ctypes.c_char_p]
     L.spoa_batch_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchStats)]
     L.spoa_batch_consensus.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_uint32]
     L.spoa_batch_msa.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_uint64,
@@ -314,24 +326,59 @@ class WindowResults:
         self.stats = stats
 
 
+def _opts(type, m, n, g, e, q, c, strand, msa, threads, device):
+    if e is None:
+        e = g
+    if q is None:
+        q, c = g, e
+    return BatchOpts(type=type, m=m, n=n, g=g, e=e, q=q, c=c, strand=int(strand), msa=int(msa),
+                     threads=threads, device=device)
+
+
+class BatchContext:
+    """A reusable batched-form engine (device buffers and host threads kept
+    between runs) for one set of scoring options."""
+
+    def __init__(self, type, m, n, g, e=None, q=None, c=None, strand=False, msa=True, threads=0, device=0):
+        self.opts = _opts(type, m, n, g, e, q, c, strand, msa, threads, device)
+        self._h = lib().spoa_batch_context_new(ctypes.byref(self.opts))
+        if not self._h:
+            raise ValueError(_error())
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.spoa_batch_context_free(self._h)
+            self._h = None
+
+    @property
+    def device(self):
+        return lib().spoa_batch_context_device(self._h)
+
+    def run(self, windows):
+        if not isinstance(windows, Windows):
+            windows = Windows.from_lists(windows)
+        h = lib().spoa_batch_context_run(self._h, windows.n_windows, windows.win_read_off.ctypes.data,
+                                         windows.read_off.ctypes.data, windows.seqs, windows.quals)
+        return _collect(h, windows, bool(self.opts.msa), bool(self.opts.strand))
+
+
 def run_windows(windows, type, m, n, g, e=None, q=None, c=None, mode="batched", strand=False,
                 msa=True, threads=0, device=0):
     """Aligns every read of every window in order into the window's graph,
     then builds its consensus (and MSA).  mode 'batched': one launch per
     round over all windows; 'per_call': the drop-in Align, one read at a
     time."""
-    if e is None:
-        e = g
-    if q is None:
-        q, c = g, e
     if not isinstance(windows, Windows):
         windows = Windows.from_lists(windows)
-    o = BatchOpts(type=type, m=m, n=n, g=g, e=e, q=q, c=c, strand=int(strand), msa=int(msa),
-                  threads=threads, device=device)
+    o = _opts(type, m, n, g, e, q, c, strand, msa, threads, device)
     md = 1 if mode == "per_call" else 0
     h = lib().spoa_batch_run(md, ctypes.byref(o), windows.n_windows,
                              windows.win_read_off.ctypes.data, windows.read_off.ctypes.data,
                              windows.seqs, windows.quals)
+    return _collect(h, windows, msa, strand)
+
+
+def _collect(h, windows, msa, strand):
     if not h:
         raise ValueError(_error())
     try:

@@ -116,14 +116,24 @@ void ThreadPool::For(std::size_t n, const std::function<void(std::size_t)>& fn) 
 
 // --------------------------------------------------------------- drivers --
 
+BatchContext::BatchContext(const BatchOptions& opt)
+    : opt_(opt),
+      engine_(spoa::AlignmentEngine::Create(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c)),
+      pool_(opt.threads) {}
+
 std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats) {
+  BatchContext ctx(opt);
+  return RunBatched(ws, ctx, stats);
+}
+
+std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, BatchStats* stats) {
   const auto t0 = Clock::now();
   BatchStats st;
-  auto engine = spoa::AlignmentEngine::Create(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c);
-  auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(engine.get());
+  const BatchOptions& opt = ctx.options();
+  auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(ctx.engine());
   if (!hip) throw std::invalid_argument("[spoa::hip] the alignment engine is not the HIP engine");
   spoa_hip_engine* dev = hip->device();
-  ThreadPool pool(opt.threads);
+  ThreadPool& pool = ctx.pool();
 
   const std::uint32_t W = ws.n_windows;
   std::vector<spoa::Graph> graphs(W);
@@ -183,6 +193,10 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& op
       st.kernel_ms += hs.kernel_ms;
       st.h2d_ms += hs.h2d_ms;
       st.d2h_ms += hs.d2h_ms;
+      st.pack_ms += hs.host_ms;
+      st.alloc_ms += hs.alloc_ms;
+      st.wait_ms += hs.wait_ms;
+      st.unpack_ms += hs.unpack_ms;
       st.cells += hs.cells;
       st.dp_bytes += hs.dp_bytes;
       st.launches += hs.launches;

@@ -75,6 +75,7 @@ struct BatchStats {
   double h2d_ms = 0, d2h_ms = 0;
   double graph_ms = 0;       // AddAlignment + topological sort
   double finish_ms = 0;      // consensus + MSA
+  double pack_ms = 0, alloc_ms = 0, wait_ms = 0, unpack_ms = 0;  // inside the alignment calls
   std::uint64_t cells = 0;   // sum of V * n over the DPs
   std::uint64_t dp_bytes = 0;
   std::uint64_t align_calls = 0;
@@ -82,8 +83,24 @@ struct BatchStats {
   std::uint32_t rounds = 0;
 };
 
+// A long-lived engine and host thread pool for repeated batched runs with
+// one set of options (the engine keeps its device buffers between runs).
+class BatchContext {
+ public:
+  explicit BatchContext(const BatchOptions& opt);
+  const BatchOptions& options() const { return opt_; }
+  spoa::AlignmentEngine* engine() { return engine_.get(); }
+  ThreadPool& pool() { return pool_; }
+
+ private:
+  BatchOptions opt_;
+  std::unique_ptr<spoa::AlignmentEngine> engine_;
+  ThreadPool pool_;
+};
+
 // Batched form.  Throws std::invalid_argument with the engine's message on
 // failure.
+std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, BatchStats* stats);
 std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats);
 
 // Plain per-call form: AlignmentEngine::Create(...)->Align for every read

@@ -200,6 +200,40 @@ void* spoa_batch_run(int mode, const spoa_batch_opts* opts, uint32_t n_windows, 
 
 void spoa_batch_free(void* h) { delete static_cast<BatchHandle*>(h); }
 
+void* spoa_batch_context_new(const spoa_batch_opts* opts) {
+  try {
+    return new spoa_amd::BatchContext(Options(opts));
+  } catch (const std::exception& ex) {
+    Error(ex);
+    return nullptr;
+  }
+}
+
+void spoa_batch_context_free(void* ctx) { delete static_cast<spoa_amd::BatchContext*>(ctx); }
+
+spoa_hip_engine* spoa_batch_context_device(void* ctx) {
+  auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(static_cast<spoa_amd::BatchContext*>(ctx)->engine());
+  return hip ? hip->device() : nullptr;
+}
+
+void* spoa_batch_context_run(void* ctx, uint32_t n_windows, const uint32_t* win_read_off, const uint64_t* read_off,
+                             const char* seqs, const char* quals) {
+  try {
+    spoa_amd::WindowSet ws;
+    ws.n_windows = n_windows;
+    ws.win_read_off = win_read_off;
+    ws.read_off = read_off;
+    ws.seqs = seqs;
+    ws.quals = quals;
+    auto h = std::make_unique<BatchHandle>();
+    h->results = spoa_amd::RunBatched(ws, *static_cast<spoa_amd::BatchContext*>(ctx), &h->stats);
+    return h.release();
+  } catch (const std::exception& ex) {
+    Error(ex);
+    return nullptr;
+  }
+}
+
 int spoa_batch_get_stats(void* h, spoa_batch_stats* out) {
   const auto& s = static_cast<BatchHandle*>(h)->stats;
   out->wall_ms = s.wall_ms;
@@ -209,6 +243,10 @@ int spoa_batch_get_stats(void* h, spoa_batch_stats* out) {
   out->d2h_ms = s.d2h_ms;
   out->graph_ms = s.graph_ms;
   out->finish_ms = s.finish_ms;
+  out->pack_ms = s.pack_ms;
+  out->alloc_ms = s.alloc_ms;
+  out->wait_ms = s.wait_ms;
+  out->unpack_ms = s.unpack_ms;
   out->cells = s.cells;
   out->dp_bytes = s.dp_bytes;
   out->align_calls = s.align_calls;

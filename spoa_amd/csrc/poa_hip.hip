@@ -4,7 +4,7 @@
 //
 // Per call (AlignmentEngine::Align) one job is packed into pinned memory
 // (rank-ordered CSR, first columns, read), copied with one transfer,
-// aligned by one workgroup of up to 16 waves, and its traceback pairs come
+// aligned by one workgroup of up to 8 waves, and its traceback pairs come
 // back with one transfer.  A batch is grouped by launch configuration,
 // ordered by V * n (largest first, the queue order of the persistent
 // grid) and split into launches whose DP matrices fit the memory budget.
@@ -18,6 +18,7 @@
 #include <limits>
 #include <map>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -89,7 +90,7 @@ struct DevBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
     size = 0;
-    const uint64_t want = std::max<uint64_t>(need, need + need / 4);
+    const uint64_t want = std::max<uint64_t>(need, std::min<uint64_t>(2 * need, std::max<uint64_t>(size * 2, need + need / 4)));
     if (hipMalloc(&p, want) == hipSuccess) {
       size = want;
       return true;
@@ -120,7 +121,7 @@ struct PinBuf {
     if (p) (void)hipHostFree(p);
     p = nullptr;
     size = 0;
-    const uint64_t want = std::max<uint64_t>(need, need + need / 4);
+    const uint64_t want = std::max<uint64_t>(need, 2 * need);
     if (!HipOk(hipHostMalloc(&p, want, hipHostMallocDefault), "hipHostMalloc")) return false;
     size = want;
     return true;
@@ -138,22 +139,34 @@ struct PinBuf {
 
 using KernelFn = void (*)(KArgs);
 
+// Multi-wave workgroups have at most 8 waves (two a SIMD), which leaves
+// each lane 256 registers.
+constexpr int kMaxGroupWaves = 8;
+
 template <int SUB, typename ST, int K>
 KernelFn PickMaxT(int waves) {
   if (waves == 1) return spoa_hip_dev::PoaAlignKernel<SUB, ST, K, 64>;
-  return spoa_hip_dev::PoaAlignKernel<SUB, ST, K, 1024>;
+  if constexpr (K == 32) {
+    return nullptr;  // 32 columns a lane only in single-wave workgroups
+  } else {
+    return spoa_hip_dev::PoaAlignKernel<SUB, ST, K, 64 * kMaxGroupWaves>;
+  }
+}
+
+template <int SUB, typename ST>
+KernelFn PickK(int k, int waves) {
+  if (k == 32) return PickMaxT<SUB, ST, 32>(waves);
+  if (k == 16) return PickMaxT<SUB, ST, 16>(waves);
+  if constexpr (SUB == 2 && sizeof(ST) == 2) {
+    return nullptr;  // convex int16 lanes must hold whole 16-column segments
+  } else {
+    return PickMaxT<SUB, ST, 8>(waves);
+  }
 }
 
 template <int SUB>
 KernelFn PickSub(int width_bits, int k, int waves) {
-  if (width_bits == 16) {
-    if constexpr (SUB == 2) {
-      return PickMaxT<SUB, int16_t, 16>(waves);
-    } else {
-      return k == 8 ? PickMaxT<SUB, int16_t, 8>(waves) : PickMaxT<SUB, int16_t, 16>(waves);
-    }
-  }
-  return k == 8 ? PickMaxT<SUB, int32_t, 8>(waves) : PickMaxT<SUB, int32_t, 16>(waves);
+  return width_bits == 16 ? PickK<SUB, int16_t>(k, waves) : PickK<SUB, int32_t>(k, waves);
 }
 
 KernelFn PickKernel(int subtype, int width_bits, int k, int waves) {
@@ -243,14 +256,22 @@ void FirstColumns(const spoa_hip_params& p, const spoa_hip_graph& gr, int32_t* f
 
 }  // namespace
 
+constexpr int kStreams = 4;
+
 struct spoa_hip_engine {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;            // copies, counters and the first group
+  hipStream_t side[kStreams - 1] = {};     // further launch groups of a pass
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  hipEvent_t ev_group[kStreams] = {};
   spoa_hip_params prm{};
   int num_cus = 256;
   uint64_t budget = 0;
-  DevBuf d_in, d_out, d_arena, d_counter;
+  DevBuf d_in, d_out, d_counter;
+  // DP matrices: grow-only segments kept for the engine's lifetime, so a
+  // long-lived engine allocates once (allocation time grows with size)
+  std::vector<DevBuf> arena;
+  uint64_t arena_bytes = 0;
   PinBuf h_in, h_out;
   spoa_hip_stats stats{};
   std::map<std::pair<KernelFn, int>, int> occupancy;
@@ -260,16 +281,27 @@ struct spoa_hip_engine {
 
 namespace {
 
+// Batched launches favour one wave per job (rows up to 64 * 32 columns),
+// the per-call launch the widest workgroup (up to 16 waves) of short lanes.
 Cfg DefaultCfg(const spoa_hip_engine* E, uint32_t W, int width_bits, bool batch) {
   Cfg c;
   const bool convex16 = E->prm.subtype == 2 && width_bits == 16;
+  const int kmin = convex16 ? 16 : 8;
   if (batch) {
-    c.k = (convex16 || W > 512) ? 16 : 8;
+    for (int k = kmin; k <= 32; k *= 2) {
+      if (W <= 64u * static_cast<uint32_t>(k)) {
+        c.k = k;
+        c.waves = 1;
+        return c;
+      }
+    }
+    c.k = 16;
   } else {
-    c.k = convex16 ? 16 : 8;
+    c.k = kmin;
   }
   const uint32_t per_wave = 64u * static_cast<uint32_t>(c.k);
-  c.waves = static_cast<int>(std::min<uint32_t>(16u, std::max<uint32_t>(1u, (W + per_wave - 1) / per_wave)));
+  c.waves = static_cast<int>(
+      std::min<uint32_t>(kMaxGroupWaves, std::max<uint32_t>(1u, (W + per_wave - 1) / per_wave)));
   return c;
 }
 
@@ -298,18 +330,61 @@ uint64_t MatBytes(const spoa_hip_params& p, uint32_t V, uint32_t stride, int wid
   return RoundUp(static_cast<uint64_t>(V + 1) * Matrices(p.subtype) * stride * (width / 8), kAlign);
 }
 
-// Packs, launches and unpacks one chunk of jobs that share a configuration.
-int RunChunk(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& plans, size_t lo, size_t hi) {
+// Packs one job's graph block and read into the staging buffer.
+void PackJob(const spoa_hip_params& P, const spoa_hip_job& j, const Plan& pl, uint32_t* g, uint8_t* s) {
+  const spoa_hip_graph& gr = j.graph;
+  const uint32_t V = gr.num_nodes;
+  uint32_t* rows = g;
+  int32_t* node_id = reinterpret_cast<int32_t*>(g + 4 * V);
+  uint32_t* poff = g + 5 * V;
+  int32_t* fcH = reinterpret_cast<int32_t*>(g + 6 * V + 1);
+  int32_t* fcF = fcH + (V + 1);
+  int32_t* fcO = fcF + (V + 1);
+  uint32_t* preds = g + 9 * V + 4;
+  const uint32_t base = gr.pred_off[0];
+  FirstColumns(P, gr, fcH, fcF, fcO);
+  for (uint32_t r = 0; r < V; ++r) {
+    const uint32_t pb = gr.pred_off[r] - base, pe = gr.pred_off[r + 1] - base;
+    const uint32_t npred = pe > pb ? pe - pb : 1u;
+    node_id[r] = gr.node_id[r];
+    poff[r] = pb;
+    rows[4 * r + 0] = static_cast<uint32_t>(gr.row_char[r]) | (gr.is_sink[r] ? 0x100u : 0u) | (npred << 9);
+    rows[4 * r + 1] = pe > pb ? gr.pred_rank[gr.pred_off[r]] + 1 : 0u;
+    rows[4 * r + 2] = pb;
+    rows[4 * r + 3] = static_cast<uint32_t>(fcH[r + 1]);
+  }
+  poff[V] = gr.pred_off[V] - base;
+  for (uint32_t k = base; k < gr.pred_off[V]; ++k) preds[k - base] = gr.pred_rank[k] + 1;
+  std::memcpy(s, j.seq, j.len);
+  std::memset(s + j.len, 0, pl.seq_bytes - j.len);
+}
+
+KernelFn GroupKernel(spoa_hip_engine* E, const Plan& pl) {
+  return PickKernel(E->prm.subtype, pl.width, pl.cfg.k, pl.cfg.waves);
+}
+
+bool SameGroup(const Plan& a, const Plan& b) {
+  return a.width == b.width && a.cfg.k == b.cfg.k && a.cfg.waves == b.cfg.waves;
+}
+
+// One pass: the jobs plans[lo, hi) (sorted by launch group, largest first
+// inside a group) are packed into one staging block, copied in one
+// transfer, aligned by one persistent launch per group -- the groups on
+// separate streams, so they share the device -- and copied back in one
+// transfer.
+int RunPass(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& plans, size_t lo, size_t hi) {
   const spoa_hip_params& P = E->prm;
-  const int width = plans[lo].width;
-  const Cfg cfg = plans[lo].cfg;
   const uint32_t nj = static_cast<uint32_t>(hi - lo);
 
-  // layout of the input block: jobs | graphs | sequences
-  uint64_t graph_words = 0, seq_bytes = 0, mat_elems = 0, pairs = 0;
+  uint64_t graph_words = 0, seq_bytes = 0, mat_bytes = 0, pairs = 0;
+  std::vector<uint64_t> g_at(nj), s_at(nj), p_at(nj);
   for (size_t x = lo; x < hi; ++x) {
+    g_at[x - lo] = graph_words;
+    s_at[x - lo] = seq_bytes;
+    p_at[x - lo] = pairs;
     graph_words += plans[x].graph_words;
     seq_bytes += plans[x].seq_bytes;
+    mat_bytes += plans[x].mat_bytes;
     pairs += jobs[plans[x].job].graph.num_nodes + jobs[plans[x].job].len;
   }
   const uint64_t off_jobs = 0;
@@ -320,102 +395,184 @@ int RunChunk(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pl
   const uint64_t off_pairs = RoundUp(static_cast<uint64_t>(nj) * 16, kAlign);
   const uint64_t out_bytes = off_pairs + pairs * 8;
 
+  const auto t_alloc0 = std::chrono::steady_clock::now();
   if (!E->h_in.Reserve(in_bytes) || !E->h_out.Reserve(out_bytes)) return SPOA_HIP_DEVICE_ERROR;
-  if (!E->d_in.Reserve(in_bytes, kMsgNoMemory) || !E->d_out.Reserve(out_bytes, kMsgNoMemory)) {
+  if (!E->d_in.Reserve(in_bytes, kMsgNoMemory) || !E->d_out.Reserve(out_bytes, kMsgNoMemory) ||
+      !E->d_counter.Reserve(256 * sizeof(uint32_t), kMsgNoMemory)) {
     return SPOA_HIP_NO_MEMORY;
   }
+  // place every job's matrices in an arena segment (first fit in order)
+  std::vector<uint64_t> addr(nj);
+  {
+    std::vector<uint64_t> used(E->arena.size(), 0);
+    size_t x = 0;
+    bool retried = false;
+    while (x < nj) {
+      const uint64_t need = plans[lo + x].mat_bytes;
+      bool placed = false;
+      for (size_t sgi = 0; sgi < E->arena.size(); ++sgi) {
+        if (used[sgi] + need <= E->arena[sgi].size) {
+          addr[x] = reinterpret_cast<uint64_t>(E->arena[sgi].p) + used[sgi];
+          used[sgi] += need;
+          placed = true;
+          break;
+        }
+      }
+      if (placed) {
+        ++x;
+        continue;
+      }
+      uint64_t rest = 0;
+      for (size_t y = x; y < nj; ++y) rest += plans[lo + y].mat_bytes;
+      const uint64_t room = E->budget > E->arena_bytes ? E->budget - E->arena_bytes : 0;
+      uint64_t want = std::min(room, std::max<uint64_t>(rest, 4ull << 30));
+      if (want < need) {
+        // fragmented: start over with one segment holding the whole pass
+        if (retried) return Fail(SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+        retried = true;
+        for (auto& sg : E->arena) sg.Release();
+        E->arena.clear();
+        E->arena_bytes = 0;
+        used.clear();
+        x = 0;
+        continue;
+      }
+      DevBuf sg;
+      if (hipMalloc(&sg.p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        want = rest;
+        if (want < need || hipMalloc(&sg.p, want) != hipSuccess) {
+          (void)hipGetLastError();
+          return Fail(SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+        }
+      }
+      sg.size = want;
+      E->arena.push_back(sg);
+      E->arena_bytes += want;
+      used.push_back(0);
+    }
+  }
+  E->stats.alloc_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_alloc0).count();
 
+  // ---- pack (host threads for large passes)
   const auto t_host0 = std::chrono::steady_clock::now();
   DevJob* dj = E->h_in.At<DevJob>(off_jobs);
-  uint64_t gw = 0, sb = 0, pr = 0, me = 0;
-  for (size_t x = lo; x < hi; ++x) {
-    const Plan& pl = plans[x];
-    const spoa_hip_job& j = jobs[pl.job];
-    const spoa_hip_graph& gr = j.graph;
-    const uint32_t V = gr.num_nodes;
-    DevJob& d = dj[x - lo];
-    d.mat_off = me / (width / 8);
-    d.graph_off = static_cast<uint32_t>(gw);
-    d.seq_off = static_cast<uint32_t>(sb);
-    d.V = V;
-    d.n = j.len;
-    d.W = pl.W;
-    d.stride = pl.stride;
-    d.pair_off = static_cast<uint32_t>(pr);
-    d.cap = V + j.len;
-    uint32_t* g = E->h_in.At<uint32_t>(off_graph) + gw;
-    int32_t* node_id = reinterpret_cast<int32_t*>(g);
-    uint32_t* info = g + V;
-    uint32_t* poff = g + 2 * V;
-    int32_t* fcH = reinterpret_cast<int32_t*>(g + 3 * V + 1);
-    int32_t* fcF = fcH + (V + 1);
-    int32_t* fcO = fcF + (V + 1);
-    uint32_t* preds = g + 6 * V + 4;
-    const uint32_t base = gr.pred_off[0];
-    for (uint32_t r = 0; r < V; ++r) {
-      node_id[r] = gr.node_id[r];
-      info[r] = static_cast<uint32_t>(gr.row_char[r]) | (gr.is_sink[r] ? 0x100u : 0u);
-      poff[r] = gr.pred_off[r] - base;
+  auto pack_range = [&](size_t a, size_t b) {
+    for (size_t x = a; x < b; ++x) {
+      const Plan& pl = plans[lo + x];
+      const spoa_hip_job& j = jobs[pl.job];
+      DevJob& d = dj[x];
+      d.mat_addr = addr[x];
+      d.graph_off = static_cast<uint32_t>(g_at[x]);
+      d.seq_off = static_cast<uint32_t>(s_at[x]);
+      d.V = j.graph.num_nodes;
+      d.n = j.len;
+      d.W = pl.W;
+      d.stride = pl.stride;
+      d.pair_off = static_cast<uint32_t>(p_at[x]);
+      d.cap = j.graph.num_nodes + j.len;
+      PackJob(P, j, pl, E->h_in.At<uint32_t>(off_graph) + g_at[x], E->h_in.At<uint8_t>(off_seq) + s_at[x]);
     }
-    poff[V] = gr.pred_off[V] - base;
-    for (uint32_t k = base; k < gr.pred_off[V]; ++k) preds[k - base] = gr.pred_rank[k] + 1;
-    FirstColumns(P, gr, fcH, fcF, fcO);
-    uint8_t* s = E->h_in.At<uint8_t>(off_seq) + sb;
-    std::memcpy(s, j.seq, j.len);
-    std::memset(s + j.len, 0, pl.seq_bytes - j.len);
-    gw += pl.graph_words;
-    sb += pl.seq_bytes;
-    pr += d.cap;
-    me += pl.mat_bytes;
-  }
-  mat_elems = me;
-  if (!E->d_arena.Reserve(mat_elems, kMsgNoMemory)) return SPOA_HIP_NO_MEMORY;
-  if (!E->d_counter.Reserve(256, kMsgNoMemory)) return SPOA_HIP_NO_MEMORY;
-  const auto t_host1 = std::chrono::steady_clock::now();
-  E->stats.host_ms += std::chrono::duration<double, std::milli>(t_host1 - t_host0).count();
-
-  KernelFn fn = PickKernel(P.subtype, width, cfg.k, cfg.waves);
-  const int threads = 64 * cfg.waves;
-  int per_cu = 0;
-  auto key = std::make_pair(fn, threads);
-  auto oc = E->occupancy.find(key);
-  if (oc == E->occupancy.end()) {
-    if (!HipOk(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0), "occupancy")) {
-      return SPOA_HIP_DEVICE_ERROR;
+  };
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const unsigned nt = nj >= 64 ? hw : 1u;
+  if (nt > 1) {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t) {
+      th.emplace_back(pack_range, static_cast<size_t>(nj) * t / nt, static_cast<size_t>(nj) * (t + 1) / nt);
     }
-    E->occupancy[key] = per_cu;
+    for (auto& t : th) t.join();
   } else {
-    per_cu = oc->second;
+    pack_range(0, nj);
   }
-  const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(nj, static_cast<uint32_t>(std::max(1, per_cu)) *
-                                                                          static_cast<uint32_t>(E->num_cus)));
+  E->stats.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
 
-  KArgs args{};
-  args.prm = MakeParams(P, width, cfg.k);
+  // ---- launch groups
+  struct Group {
+    size_t a, b;  // job range inside the pass
+    KernelFn fn;
+    int threads;
+    uint32_t grid;
+    KParams prm;
+  };
+  std::vector<Group> groups;
+  for (size_t x = lo; x < hi;) {
+    size_t y = x;
+    while (y < hi && SameGroup(plans[y], plans[x])) ++y;
+    Group gp;
+    gp.a = x - lo;
+    gp.b = y - lo;
+    gp.fn = GroupKernel(E, plans[x]);
+    gp.threads = 64 * plans[x].cfg.waves;
+    if (!gp.fn) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] unsupported launch configuration");
+    int per_cu = 0;
+    auto key = std::make_pair(gp.fn, gp.threads);
+    auto oc = E->occupancy.find(key);
+    if (oc == E->occupancy.end()) {
+      if (!HipOk(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gp.fn, gp.threads, 0), "occupancy")) {
+        return SPOA_HIP_DEVICE_ERROR;
+      }
+      E->occupancy[key] = per_cu;
+    } else {
+      per_cu = oc->second;
+    }
+    const uint32_t n = static_cast<uint32_t>(gp.b - gp.a);
+    gp.grid = std::max<uint32_t>(
+        1u, std::min<uint32_t>(n, static_cast<uint32_t>(std::max(1, per_cu)) * static_cast<uint32_t>(E->num_cus)));
+    gp.prm = MakeParams(P, plans[x].width, plans[x].cfg.k);
+    groups.push_back(gp);
+    x = y;
+  }
+  if (groups.size() > 256) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] too many launch groups");
+
   uint8_t* din = static_cast<uint8_t*>(E->d_in.p);
   uint8_t* dout = static_cast<uint8_t*>(E->d_out.p);
-  args.jobs = reinterpret_cast<const DevJob*>(din + off_jobs);
-  args.n_jobs = nj;
-  args.queue = static_cast<uint32_t*>(E->d_counter.p);
-  args.graphs = reinterpret_cast<const uint32_t*>(din + off_graph);
-  args.seqs = din + off_seq;
-  args.arena = E->d_arena.p;
-  args.pairs = reinterpret_cast<int32_t*>(dout + off_pairs);
-  args.results = reinterpret_cast<int32_t*>(dout + off_res);
-
+  uint32_t* counters = static_cast<uint32_t*>(E->d_counter.p);
   hipStream_t st = E->stream;
   if (!HipOk(hipEventRecord(E->ev0, st), "event") ||
-      !HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in") ||
-      !HipOk(hipEventRecord(E->ev1, st), "event")) {
+      !HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in")) {
     return SPOA_HIP_DEVICE_ERROR;
   }
-  spoa_hip_dev::ResetCounter<<<1, 64, 0, st>>>(args.queue);
-  fn<<<grid, threads, 0, st>>>(args);
-  if (!HipOk(hipGetLastError(), "kernel launch") || !HipOk(hipEventRecord(E->ev2, st), "event") ||
+  spoa_hip_dev::ResetCounters<<<1, 256, 0, st>>>(counters, static_cast<uint32_t>(groups.size()));
+  if (!HipOk(hipGetLastError(), "counter reset") || !HipOk(hipEventRecord(E->ev1, st), "event")) {
+    return SPOA_HIP_DEVICE_ERROR;
+  }
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    const Group& gp = groups[gi];
+    const int si = static_cast<int>(gi % kStreams);
+    hipStream_t gs = si == 0 ? st : E->side[si - 1];
+    if (si != 0 && gi < kStreams && !HipOk(hipStreamWaitEvent(gs, E->ev1, 0), "stream wait")) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+    KArgs args{};
+    args.prm = gp.prm;
+    args.jobs = reinterpret_cast<const DevJob*>(din + off_jobs) + gp.a;
+    args.n_jobs = static_cast<uint32_t>(gp.b - gp.a);
+    args.queue = counters + gi;
+    args.graphs = reinterpret_cast<const uint32_t*>(din + off_graph);
+    args.seqs = din + off_seq;
+    args.pairs = reinterpret_cast<int32_t*>(dout + off_pairs);
+    args.results = reinterpret_cast<int32_t*>(dout + off_res) + 4 * gp.a;
+    gp.fn<<<gp.grid, gp.threads, 0, gs>>>(args);
+    if (!HipOk(hipGetLastError(), "kernel launch")) return SPOA_HIP_DEVICE_ERROR;
+    E->stats.launches += 1;
+  }
+  // join the side streams
+  for (int si = 1; si < kStreams && si < static_cast<int>(groups.size()); ++si) {
+    if (!HipOk(hipEventRecord(E->ev_group[si], E->side[si - 1]), "event") ||
+        !HipOk(hipStreamWaitEvent(st, E->ev_group[si], 0), "stream wait")) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+  }
+  if (!HipOk(hipEventRecord(E->ev2, st), "event") ||
       !HipOk(hipMemcpyAsync(E->h_out.p, E->d_out.p, out_bytes, hipMemcpyDeviceToHost, st), "copy out") ||
-      !HipOk(hipEventRecord(E->ev3, st), "event") || !HipOk(hipStreamSynchronize(st), "kernel")) {
+      !HipOk(hipEventRecord(E->ev3, st), "event")) {
     return SPOA_HIP_DEVICE_ERROR;
   }
+  const auto t_wait0 = std::chrono::steady_clock::now();
+  if (!HipOk(hipStreamSynchronize(st), "kernel")) return SPOA_HIP_DEVICE_ERROR;
+  const auto t_wait1 = std::chrono::steady_clock::now();
+  E->stats.wait_ms += std::chrono::duration<double, std::milli>(t_wait1 - t_wait0).count();
   float ms_in = 0, ms_k = 0, ms_out = 0;
   (void)hipEventElapsedTime(&ms_in, E->ev0, E->ev1);
   (void)hipEventElapsedTime(&ms_k, E->ev1, E->ev2);
@@ -423,9 +580,8 @@ int RunChunk(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pl
   E->stats.h2d_ms += ms_in;
   E->stats.kernel_ms += ms_k;
   E->stats.d2h_ms += ms_out;
-  E->stats.launches += 1;
 
-  // unpack: pairs come back in traceback order
+  // ---- unpack: pairs come back in traceback order
   const int32_t* res = E->h_out.At<int32_t>(off_res);
   const int32_t* allp = E->h_out.At<int32_t>(off_pairs);
   int first = SPOA_HIP_OK;
@@ -433,7 +589,7 @@ int RunChunk(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pl
     spoa_hip_job& j = jobs[plans[x].job];
     const DevJob& d = dj[x - lo];
     const int32_t* r4 = res + 4 * (x - lo);
-    j.width_bits = width;
+    j.width_bits = plans[x].width;
     j.has_score = r4[2];
     j.score = r4[1];
     const uint32_t np = static_cast<uint32_t>(r4[0]);
@@ -458,6 +614,7 @@ int RunChunk(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pl
     }
     j.status = status;
   }
+  E->stats.unpack_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wait1).count();
   return first;
 }
 
@@ -503,7 +660,7 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     pl.stride = static_cast<uint32_t>(RoundUp(pl.W, static_cast<uint64_t>(pl.cfg.k)));
     pl.mat_bytes = MatBytes(P, V, pl.stride, width);
     const uint32_t E_in = j.graph.pred_off[V] - j.graph.pred_off[0];
-    pl.graph_words = RoundUp(6ull * V + 4 + E_in, 4);
+    pl.graph_words = RoundUp(9ull * V + 4 + E_in, 4);
     pl.seq_bytes = RoundUp(pl.stride + 16, 16);
     if (pl.mat_bytes > E->budget) {
       fail(j, SPOA_HIP_NO_MEMORY, kMsgNoMemory);
@@ -531,12 +688,11 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
   while (lo < plans.size()) {
     size_t hi = lo;
     uint64_t bytes = 0;
-    while (hi < plans.size() && plans[hi].width == plans[lo].width && plans[hi].cfg.k == plans[lo].cfg.k &&
-           plans[hi].cfg.waves == plans[lo].cfg.waves && bytes + plans[hi].mat_bytes <= E->budget) {
+    while (hi < plans.size() && bytes + plans[hi].mat_bytes <= E->budget) {
       bytes += plans[hi].mat_bytes;
       ++hi;
     }
-    const int rc = RunChunk(E, jobs, plans, lo, hi);
+    const int rc = RunPass(E, jobs, plans, lo, hi);
     if (rc != SPOA_HIP_OK && first == SPOA_HIP_OK) first = rc;
     if (rc == SPOA_HIP_DEVICE_ERROR || rc == SPOA_HIP_NO_MEMORY) {
       const std::string msg = g_last_error;
@@ -584,6 +740,18 @@ int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_e
     spoa_hip_engine_destroy(E);
     return SPOA_HIP_DEVICE_ERROR;
   }
+  for (int i = 0; i < kStreams - 1; ++i) {
+    if (!HipOk(hipStreamCreateWithFlags(&E->side[i], hipStreamNonBlocking), "hipStreamCreate")) {
+      spoa_hip_engine_destroy(E);
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+  }
+  for (int i = 0; i < kStreams; ++i) {
+    if (!HipOk(hipEventCreateWithFlags(&E->ev_group[i], hipEventDisableTiming), "hipEventCreate")) {
+      spoa_hip_engine_destroy(E);
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+  }
   *out = E;
   return SPOA_HIP_OK;
 }
@@ -594,7 +762,8 @@ int spoa_hip_engine_destroy(spoa_hip_engine* E) {
   if (E->stream) (void)hipStreamSynchronize(E->stream);
   E->d_in.Release();
   E->d_out.Release();
-  E->d_arena.Release();
+  for (auto& sg : E->arena) sg.Release();
+  E->arena.clear();
   E->d_counter.Release();
   E->h_in.Release();
   E->h_out.Release();
@@ -602,6 +771,12 @@ int spoa_hip_engine_destroy(spoa_hip_engine* E) {
   if (E->ev1) (void)hipEventDestroy(E->ev1);
   if (E->ev2) (void)hipEventDestroy(E->ev2);
   if (E->ev3) (void)hipEventDestroy(E->ev3);
+  for (int i = 0; i < kStreams - 1; ++i) {
+    if (E->side[i]) (void)hipStreamDestroy(E->side[i]);
+  }
+  for (int i = 0; i < kStreams; ++i) {
+    if (E->ev_group[i]) (void)hipEventDestroy(E->ev_group[i]);
+  }
   if (E->stream) (void)hipStreamDestroy(E->stream);
   delete E;
   return SPOA_HIP_OK;
@@ -624,8 +799,18 @@ int spoa_hip_prealloc(spoa_hip_engine* E, uint32_t max_len, uint32_t alphabet_si
   const uint64_t stride = RoundUp(W, static_cast<uint64_t>(cfg.k));
   const uint64_t bytes =
       RoundUp(static_cast<uint64_t>(V + 1) * Matrices(E->prm.subtype) * stride * (width / 8), kAlign);
-  if (bytes > E->budget || !E->d_arena.Reserve(bytes, kMsgPreNoMemory)) {
-    return Fail(SPOA_HIP_NO_MEMORY, kMsgPreNoMemory);
+  if (bytes > E->budget) return Fail(SPOA_HIP_NO_MEMORY, kMsgPreNoMemory);
+  uint64_t largest = 0;
+  for (const auto& sg : E->arena) largest = std::max(largest, sg.size);
+  if (largest < bytes) {
+    DevBuf sg;
+    if (hipMalloc(&sg.p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return Fail(SPOA_HIP_NO_MEMORY, kMsgPreNoMemory);
+    }
+    sg.size = bytes;
+    E->arena.push_back(sg);
+    E->arena_bytes += bytes;
   }
   return SPOA_HIP_OK;
 }
@@ -679,8 +864,8 @@ int spoa_hip_set_config(spoa_hip_engine* E, int32_t width_bits, int32_t batch, u
     table.clear();
     return SPOA_HIP_OK;
   }
-  if ((k != 8 && k != 16) || waves < 1 || waves > spoa_hip_dev::kMaxWaves ||
-      (E->prm.subtype == 2 && width_bits == 16 && k != 16)) {
+  if ((k != 8 && k != 16 && k != 32) || waves < 1 || waves > kMaxGroupWaves ||
+      (E->prm.subtype == 2 && width_bits == 16 && k == 8) || (k == 32 && waves != 1)) {
     return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] unsupported launch configuration");
   }
   Cfg c;

@@ -53,7 +53,7 @@ struct KParams {
 
 // One (graph, read) job.  Offsets are in elements of the respective buffer.
 struct DevJob {
-  uint64_t mat_off;    // score-type elements into the DP arena
+  uint64_t mat_addr;   // device address of the job's DP matrices
   uint32_t graph_off;  // 32-bit words into the graph buffer
   uint32_t seq_off;    // bytes into the sequence buffer
   uint32_t V;          // graph nodes
@@ -64,13 +64,16 @@ struct DevJob {
   uint32_t cap;        // pair capacity (V + n suffices)
 };
 
-// Graph block of a job (32-bit words from graph_off):
-//   node_id[V], info[V] (decoded base | sink << 8), pred_off[V + 1]
-//   (relative to preds), fcH[V + 1], fcF[V + 1], fcO[V + 1], preds[...]
-//   (predecessor rows, rank + 1, in Node::inedges order).
+// Graph block of a job (32-bit words from graph_off, 16-byte aligned):
+//   rows[V] (uint4 per rank: decoded base | sink << 8 | predecessor count
+//   << 9 (1 for a source, whose predecessor is row 0), first predecessor
+//   row, offset of the predecessor list in preds, fcH of the row),
+//   node_id[V], pred_off[V + 1] (relative to preds), fcH[V + 1],
+//   fcF[V + 1], fcO[V + 1], preds[...] (predecessor rows, rank + 1, in
+//   Node::inedges order).
 struct GraphView {
+  const uint4* rows;
   const int32_t* node_id;
-  const uint32_t* info;
   const uint32_t* pred_off;
   const int32_t* fcH;
   const int32_t* fcF;
@@ -80,13 +83,13 @@ struct GraphView {
 
 __device__ __forceinline__ GraphView MakeView(const uint32_t* base, uint32_t V) {
   GraphView g;
-  g.node_id = reinterpret_cast<const int32_t*>(base);
-  g.info = base + V;
-  g.pred_off = base + 2 * V;
-  g.fcH = reinterpret_cast<const int32_t*>(base + 3 * V + 1);
+  g.rows = reinterpret_cast<const uint4*>(base);
+  g.node_id = reinterpret_cast<const int32_t*>(base + 4 * V);
+  g.pred_off = base + 5 * V;
+  g.fcH = reinterpret_cast<const int32_t*>(base + 6 * V + 1);
   g.fcF = g.fcH + (V + 1);
   g.fcO = g.fcF + (V + 1);
-  g.preds = base + 6 * V + 4;
+  g.preds = base + 9 * V + 4;
   return g;
 }
 
@@ -97,7 +100,6 @@ struct KArgs {
   uint32_t* queue;           // job counter, zero at launch
   const uint32_t* graphs;
   const uint8_t* seqs;
-  void* arena;
   int32_t* pairs;            // (node id, column) pairs in traceback order
   int32_t* results;          // per job: n_pairs, score, has_score, status
 };
@@ -215,7 +217,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
   };
   auto prof = [&](int32_t r, int32_t col) -> int32_t {
     if (col >= static_cast<int32_t>(n)) return prm.pad;
-    const uint32_t ch = gv.info[r - 1] & 0xffu;
+    const uint32_t ch = gv.rows[r - 1].x & 0xffu;
     return seq[col] == ch ? prm.m : prm.mis;
   };
 
@@ -382,6 +384,62 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
   *status_out = status;
 }
 
+// ------------------------------------------------------- packed rows --
+// A lane's K columns of one stored row as the 32-bit words the 16-byte
+// loads and stores move (int16: two columns a word).  The previous row is
+// kept in this form in registers, bit-identical to what memory holds.
+template <typename ST, int K>
+constexpr int Words() {
+  return sizeof(ST) == 2 ? K / 2 : K;
+}
+
+template <typename ST, int K>
+__device__ __forceinline__ void Pack(const int32_t (&v)[K], uint32_t (&w)[Words<ST, K>()]) {
+  if constexpr (sizeof(ST) == 2) {
+#pragma unroll
+    for (int i = 0; i < K / 2; ++i) {
+      w[i] = (static_cast<uint32_t>(v[2 * i]) & 0xffffu) | (static_cast<uint32_t>(v[2 * i + 1]) << 16);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < K; ++i) w[i] = static_cast<uint32_t>(v[i]);
+  }
+}
+
+template <typename ST, int K>
+__device__ __forceinline__ void Unpack(const uint32_t (&w)[Words<ST, K>()], int32_t (&v)[K]) {
+  if constexpr (sizeof(ST) == 2) {
+#pragma unroll
+    for (int i = 0; i < K / 2; ++i) {
+      v[2 * i] = static_cast<int32_t>(static_cast<int16_t>(w[i] & 0xffffu));
+      v[2 * i + 1] = static_cast<int32_t>(w[i]) >> 16;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < K; ++i) v[i] = static_cast<int32_t>(w[i]);
+  }
+}
+
+template <typename ST, int K>
+__device__ __forceinline__ void LoadWords(const ST* __restrict__ p, uint32_t (&w)[Words<ST, K>()]) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+  for (int i = 0; i < Words<ST, K>() / 4; ++i) {
+    const uint4 u = q[i];
+    w[4 * i + 0] = u.x;
+    w[4 * i + 1] = u.y;
+    w[4 * i + 2] = u.z;
+    w[4 * i + 3] = u.w;
+  }
+}
+
+template <typename ST, int K>
+__device__ __forceinline__ void StoreWords(ST* __restrict__ p, const uint32_t (&w)[Words<ST, K>()]) {
+  uint4* q = reinterpret_cast<uint4*>(p);
+#pragma unroll
+  for (int i = 0; i < Words<ST, K>() / 4; ++i) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
 // --------------------------------------------------------------- kernel --
 // SUB: 0 linear, 1 affine, 2 convex.  ST: stored score type.  K: columns
 // per lane.  MAXT: launch bound (threads per workgroup).
@@ -389,7 +447,9 @@ template <int SUB, typename ST, int K, int MAXT>
 __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
   constexpr int L = sizeof(ST) == 2 ? 16 : 8;
   constexpr int NM = SUB == 0 ? 1 : (SUB == 1 ? 3 : 5);  // H | F E | O Q
+  constexpr int KW = Words<ST, K>();
   static_assert(SUB != 2 || K % L == 0, "convex lanes must start reference segments");
+  static_assert(KW % 4 == 0, "rows move in 16-byte accesses");
   constexpr int32_t kMinI = -2147483647 - 1;
 
   __shared__ int32_t s_tot[2][kMaxWaves][2];
@@ -416,26 +476,23 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
     const uint32_t V = Uniform(J.V), n = Uniform(J.n), W = Uniform(J.W), stride = Uniform(J.stride);
     const GraphView gv = MakeView(a.graphs + J.graph_off, V);
     const uint8_t* __restrict__ seq = a.seqs + J.seq_off;
-    ST* __restrict__ mat = reinterpret_cast<ST*>(a.arena) + J.mat_off;
+    ST* __restrict__ mat = reinterpret_cast<ST*>(J.mat_addr);
     const uint64_t rstride = static_cast<uint64_t>(stride) * NM;
     const uint32_t tile_w = static_cast<uint32_t>(nwaves) * 64u * K;
     const uint32_t ntiles = (stride + tile_w - 1) / tile_w;
+    const bool single = ntiles == 1;  // a row is one tile: the previous row stays in registers
     const bool sync = nwaves > 1 || ntiles > 1;
 
     // row 0 (:556-694): E = g + c*e, Q = q + c*c, F = O = -inf, H per type
+    auto row0_h = [&](int32_t ci) -> int32_t {
+      if (type == 0) return 0;
+      if (SUB == 0) return g + ci * g;
+      if (SUB == 1) return g + ci * e;
+      return max(g + ci * e, q + ci * cc);
+    };
     for (uint32_t col = tid; col < stride; col += blockDim.x) {
       const int32_t ci = static_cast<int32_t>(col);
-      int32_t h;
-      if (type == 0) {
-        h = 0;
-      } else if (SUB == 0) {
-        h = g + ci * g;
-      } else if (SUB == 1) {
-        h = g + ci * e;
-      } else {
-        h = max(g + ci * e, q + ci * cc);
-      }
-      mat[col] = static_cast<ST>(h);
+      mat[col] = static_cast<ST>(row0_h(ci));
       if (SUB >= 1) {
         mat[stride + col] = static_cast<ST>(prm.ninf);
         mat[2 * stride + col] = static_cast<ST>(g + ci * e);
@@ -457,25 +514,43 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
     uint32_t scp[K / 4];
     int32_t nvalid;
     LoadChars<K>(seq, static_cast<uint32_t>(tid) * K, n, static_cast<uint32_t>(tid) * K < stride, scp, nvalid);
+    // the previous row (H, F, O) of this lane's columns, starting with row 0
+    uint32_t cH[KW], cF[KW], cO[KW];
+    if (single) {
+      int32_t v[K], f[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        v[k] = row0_h(tid * K + k);
+        f[k] = prm.ninf;
+      }
+      Pack<ST, K>(v, cH);
+      Pack<ST, K>(f, cF);
+      Pack<ST, K>(f, cO);
+    }
+    int32_t fch_prev = gv.fcH[0];
+    uint4 rec = gv.rows[0];
 
     for (uint32_t r = 1; r <= V; ++r) {
-      const uint32_t info = gv.info[r - 1];
-      const uint32_t rch = info & 0xffu;
-      const bool sink = (info >> 8) != 0;
-      const uint32_t pb = gv.pred_off[r - 1], pe = gv.pred_off[r];
-      const bool source = pb == pe;
-      const uint32_t npred = source ? 1u : pe - pb;
+      // row record: base | sink | predecessor count, first predecessor row,
+      // offset of the predecessor list, first-column score (next one prefetched)
+      const uint4 cur = rec;
+      if (r < V) rec = gv.rows[r];
+      const uint32_t rch = cur.x & 0xffu;
+      const bool sink = ((cur.x >> 8) & 1u) != 0;
+      const uint32_t npred = cur.x >> 9;
+      const uint32_t p0 = cur.y, pb = cur.z;
+      const int32_t fch_r = static_cast<int32_t>(cur.w);
       ST* __restrict__ row = mat + static_cast<uint64_t>(r) * rstride;
       // carry entering column 0: linear H, affine E = fc + g, convex (E, Q)
-      int32_t y0 = gv.fcH[r] + g;
-      int32_t y1 = gv.fcH[r] + q;
+      int32_t y0 = fch_r + g;
+      int32_t y1 = fch_r + q;
 
       for (uint32_t t = 0; t < ntiles; ++t) {
         const uint32_t c0 = t * tile_w + static_cast<uint32_t>(tid) * K;
         const bool active = c0 < stride;
 
         // ---- 1. vertical pass over the predecessors, in in-edge order
-        if (ntiles > 1) LoadChars<K>(seq, c0, n, active, scp, nvalid);
+        if (!single) LoadChars<K>(seq, c0, n, active, scp, nvalid);
         int32_t h1[K], fv[K], ov[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -484,19 +559,27 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
           ov[k] = kMinI;
         }
         for (uint32_t pi = 0; pi < npred; ++pi) {
-          const uint32_t p = source ? 0u : Uniform(gv.preds[pb + pi]);
+          const uint32_t p = pi == 0 ? p0 : Uniform(gv.preds[pb + pi]);
           const ST* __restrict__ prow = mat + static_cast<uint64_t>(p) * rstride;
+          const bool cached = single && p + 1 == r;
           int32_t hv[K];
-          if (active) {
+          if (cached) {
+            Unpack<ST, K>(cH, hv);
+          } else if (active) {
             LoadCols<ST, K>(prow + c0, hv);
           } else {
 #pragma unroll
             for (int k = 0; k < K; ++k) hv[k] = 0;
           }
-          // H_p[c0 - 1]: neighbouring lane, or memory across waves/tiles
+          // H_p[c0 - 1]: neighbouring lane, or the first column / memory
+          // across waves and tiles
           int32_t hm1 = __shfl_up(hv[K - 1], 1, 64);
           if (lane == 0) {
-            hm1 = c0 == 0 ? gv.fcH[p] : (active ? static_cast<int32_t>(prow[c0 - 1]) : 0);
+            if (c0 == 0) {
+              hm1 = p + 1 == r ? fch_prev : gv.fcH[p];
+            } else {
+              hm1 = active ? static_cast<int32_t>(prow[c0 - 1]) : 0;
+            }
           }
           // diagonal (:838-850): H_p[c - 1] + profile
           {
@@ -512,14 +595,28 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
           if (SUB == 0) {
 #pragma unroll
             for (int k = 0; k < K; ++k) fv[k] = max(fv[k], hv[k] + g);
-          } else if (active) {
+          } else {
             int32_t fp[K];
-            LoadCols<ST, K>(prow + stride + c0, fp);
+            if (cached) {
+              Unpack<ST, K>(cF, fp);
+            } else if (active) {
+              LoadCols<ST, K>(prow + stride + c0, fp);
+            } else {
+#pragma unroll
+              for (int k = 0; k < K; ++k) fp[k] = 0;
+            }
 #pragma unroll
             for (int k = 0; k < K; ++k) fv[k] = max(fv[k], max(hv[k] + g, fp[k] + e));
             if (SUB == 2) {
               int32_t op[K];
-              LoadCols<ST, K>(prow + 3 * stride + c0, op);
+              if (cached) {
+                Unpack<ST, K>(cO, op);
+              } else if (active) {
+                LoadCols<ST, K>(prow + 3 * stride + c0, op);
+              } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k) op[k] = 0;
+              }
 #pragma unroll
               for (int k = 0; k < K; ++k) ov[k] = max(ov[k], max(hv[k] + q, op[k] + cc));
             }
@@ -676,14 +773,34 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
               if (SUB == 2) h = max(h, qv[k]);
               hs[k] = type == 0 ? max(h, 0) : h;
             }
-            StoreCols<ST, K>(row + stride + c0, fv);
-            StoreCols<ST, K>(row + 2 * stride + c0, ev);
+            uint32_t wF[KW], wE[KW];
+            Pack<ST, K>(fv, wF);
+            Pack<ST, K>(ev, wE);
+            StoreWords<ST, K>(row + stride + c0, wF);
+            StoreWords<ST, K>(row + 2 * stride + c0, wE);
+            if (single) {
+#pragma unroll
+              for (int i = 0; i < KW; ++i) cF[i] = wF[i];
+            }
             if (SUB == 2) {
-              StoreCols<ST, K>(row + 3 * stride + c0, ov);
-              StoreCols<ST, K>(row + 4 * stride + c0, qv);
+              uint32_t wO[KW], wQ[KW];
+              Pack<ST, K>(ov, wO);
+              Pack<ST, K>(qv, wQ);
+              StoreWords<ST, K>(row + 3 * stride + c0, wO);
+              StoreWords<ST, K>(row + 4 * stride + c0, wQ);
+              if (single) {
+#pragma unroll
+                for (int i = 0; i < KW; ++i) cO[i] = wO[i];
+              }
             }
           }
-          StoreCols<ST, K>(row + c0, hs);
+          uint32_t wH[KW];
+          Pack<ST, K>(hs, wH);
+          StoreWords<ST, K>(row + c0, wH);
+          if (single) {
+#pragma unroll
+            for (int i = 0; i < KW; ++i) cH[i] = wH[i];
+          }
 
           // end-cell candidates (:903-928): SW all rows, OV sink rows (row
           // maxima floored at 0, :252-262), NW sink rows at column n - 1
@@ -716,13 +833,15 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
         y1 = tile_out1;
         if (sync) __syncthreads();  // the next tile / row reads these stores
       }
+      fch_prev = fch_r;
     }
+    // the traceback reads what every wave stored
+    __threadfence_block();
 
     // ---- end cell: the best value, then the first row reaching it
     {
       int32_t v = best, rr = best_row < 0 ? 0x7fffffff : best_row;
-      // SW and OV floor at 0 and NW starts from -inf: lanes without a
-      // candidate row do not compete
+      // lanes without a candidate row do not compete
       if (best_row < 0) v = kMinI;
 #pragma unroll
       for (int st = 32; st >= 1; st >>= 1) {
@@ -795,10 +914,10 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
   }
 }
 
-// Zeroes the job counter (a kernel rather than a memset so the launch
-// sequence can be captured in a graph).
-__global__ void ResetCounter(uint32_t* p) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
+// Zeroes the job counters of a pass (a kernel rather than a memset so the
+// launch sequence can be captured in a graph).
+__global__ void ResetCounters(uint32_t* p, uint32_t n) {
+  if (blockIdx.x == 0 && threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 }  // namespace spoa_hip_dev

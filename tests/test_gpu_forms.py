@@ -1,0 +1,125 @@
+"""GPU parity of the launch configurations and of the batched form.
+
+* Every launch configuration (columns per lane, waves per workgroup, rows
+  split into several tiles) gives the same alignments as the oracle.
+* The batched form (one launch per round over many windows) gives, window
+  by window, the consensus and MSA of the compiled reference's vectors and
+  of the oracle, and the plain per-call form gives the same."""
+import gzip
+import hashlib
+import json
+import os
+
+import pytest
+
+import common
+import oracle_lib as ol
+import spoa_amd
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def vectors():
+    with gzip.open(os.path.join(common.GOLDEN, "ref_vectors.json.gz"), "rt") as f:
+        return json.load(f)
+
+
+def msa_hash(rows):
+    return hashlib.sha1("\n".join(rows).encode()).hexdigest()
+
+
+def _oracle_window(params, reads, strand=False):
+    return ol.run_window(ol.oracle(), params, reads, strand=strand)
+
+
+# (columns per lane, waves); with rows of up to ~700 columns, k=8 and one
+# wave splits a row into two tiles
+CONFIGS = [(8, 1), (16, 1), (32, 1), (8, 2), (8, 3), (16, 2), (16, 5), (8, 8)]
+
+
+@pytest.mark.parametrize("k,waves", CONFIGS)
+@pytest.mark.parametrize("mode", [0, 4, 8])
+def test_launch_configs(k, waves, mode):
+    params = common.MODES[mode]
+    data = common.sample_reads()[:24]
+    reads = [s for s, _ in data]
+    cons_o, msa_o, recs_o = _oracle_window(params, reads)
+    eng = spoa_amd.AlignmentEngine.create(*params)
+    lib = spoa_amd.lib()
+    for width in (16, 32):
+        kk = k
+        if params[3:] != (-8, -8, -8, -8) and params[3:] != (-8, -6, -8, -6) and width == 16 and k == 8:
+            kk = 16  # convex int16 lanes hold whole 16-column segments
+        assert lib.spoa_hip_set_config(eng.device, width, 0, 1 << 30, kk, waves) == 0
+    gr = spoa_amd.Graph()
+    for r, rec in zip(reads, recs_o):
+        a, s = eng.align(r, gr)
+        assert s == rec["score"]
+        assert a == rec["pairs"]
+        gr.add_alignment(a, r)
+    assert gr.generate_consensus() == cons_o
+    assert gr.generate_multiple_sequence_alignment() == msa_o
+
+
+def test_batched_matches_reference_vectors(vectors):
+    """Config-shaped windows recorded from the compiled reference."""
+    for cfg in (1, 2, 3):
+        recs = [r for r in vectors["configs"] if r["config"] == cfg]
+        ws = spoa_amd.Windows.from_lists([r["reads"] for r in recs])
+        res = spoa_amd.run_windows(ws, *recs[0]["params"], mode="batched", msa=True)
+        for w, r in enumerate(recs):
+            assert res.consensus[w] == r["consensus"], (cfg, w)
+            assert msa_hash(res.msa[w]) == r["msa_sha1"], (cfg, w)
+
+
+def test_per_call_matches_reference_vectors(vectors):
+    for cfg in (1, 2):
+        recs = [r for r in vectors["configs"] if r["config"] == cfg]
+        ws = spoa_amd.Windows.from_lists([r["reads"] for r in recs])
+        res = spoa_amd.run_windows(ws, *recs[0]["params"], mode="per_call", msa=True)
+        for w, r in enumerate(recs):
+            assert res.consensus[w] == r["consensus"]
+            assert msa_hash(res.msa[w]) == r["msa_sha1"]
+
+
+def test_strand_window_matches_reference_vectors(vectors):
+    """Long global convex window, both strands per read (int32 scores)."""
+    recs = [r for r in vectors["configs"] if r["config"] == 4]
+    ws = spoa_amd.Windows.from_lists([r["reads"] for r in recs])
+    res = spoa_amd.run_windows(ws, *recs[0]["params"], mode="batched", strand=True, msa=True)
+    for w, r in enumerate(recs):
+        assert res.consensus[w] == r["consensus"]
+        assert msa_hash(res.msa[w]) == r["msa_sha1"]
+        assert res.reversed[w] == [0 if c["kept_forward"] else 1 for c in r["calls"]]
+
+
+@pytest.mark.parametrize("cfg", [1, 3])
+def test_batched_matches_oracle_many_windows(cfg):
+    """Many windows of uneven read counts in one batch, each equal to the
+    oracle run alone."""
+    c = spoa_amd.CONFIGS[cfg]
+    ws = spoa_amd.config_windows(cfg, seed=4242, n_windows=24)
+    res = spoa_amd.run_windows(ws, *c["params"], mode="batched", msa=True)
+    for w in range(ws.n_windows):
+        cons, msa, _ = _oracle_window(c["params"], ws.reads(w))
+        assert res.consensus[w] == cons, w
+        assert res.msa[w] == msa, w
+
+
+def test_context_reuse_and_budget_split():
+    """A reused batched context gives identical results run after run, also
+    when a small memory budget splits every round into several passes."""
+    c = spoa_amd.CONFIGS[3]
+    ws = spoa_amd.config_windows(3, seed=99, n_windows=12)
+    ref = spoa_amd.run_windows(ws, *c["params"], mode="batched", msa=True)
+    ctx = spoa_amd.BatchContext(*c["params"], msa=True)
+    for _ in range(2):
+        res = ctx.run(ws)
+        assert res.consensus == ref.consensus
+        assert res.msa == ref.msa
+    # about three jobs' matrices per pass
+    assert spoa_amd.lib().spoa_hip_set_memory_budget(ctx.device, 3 * 4 * 1024 * 1024 * 3) == 0
+    res = ctx.run(ws)
+    assert res.consensus == ref.consensus
+    assert res.msa == ref.msa
