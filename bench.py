@@ -99,12 +99,22 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-call", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
               "(windows shard across GPUs without any exchange, see DESIGN.md)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cfg = spoa_amd.CONFIGS[args.config]
     n_windows = args.windows or {1: 1024, 2: 512, 3: 4096, 4: 64}[args.config]

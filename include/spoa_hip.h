@@ -104,6 +104,16 @@ int spoa_hip_align(spoa_hip_engine* engine, const spoa_hip_graph* graph, const c
  * value is the first failure. */
 int spoa_hip_align_batch(spoa_hip_engine* engine, spoa_hip_job* jobs, uint32_t n_jobs);
 
+/* The batch form split in two, so the host can work while the device
+ * aligns (no counterpart in the reference): submit packs and launches
+ * (any memory-budget passes but the last run to completion), wait blocks
+ * until the last pass is done and fills the job records; it returns what
+ * spoa_hip_align_batch would have.  The jobs, their graphs and reads must
+ * stay untouched until wait returns.  A further submit or align on the
+ * engine finishes a pending batch first. */
+int spoa_hip_align_batch_submit(spoa_hip_engine* engine, spoa_hip_job* jobs, uint32_t n_jobs);
+int spoa_hip_align_batch_wait(spoa_hip_engine* engine);
+
 /* Measurements of the last align / align_batch call: device time of the
  * alignment kernels from HIP events on the engine's stream, copies, launch
  * count, DP cells (sum of V * n) and algorithmic DP bytes (sum of

@@ -421,8 +421,8 @@ def _collect(h, windows, msa, strand):
 def synth_windows(seed, n_windows, template=(500, 500), reads=(20, 20), read_len=(0, 0),
                   err=(0.1, 0.1), revcomp_odd=False, log_uniform_len=False):
     """Seeded synthetic windows: uniform ACGT templates, reads as template
-    substrings (read as a circle) or whole templates, per-base errors split
-    evenly into substitutions, insertions and deletions."""
+    substrings (length clipped to the template) or whole templates, per-base
+    errors split evenly into substitutions, insertions and deletions."""
     sp = SynthSpec(template[0], template[1], reads[0], reads[1], read_len[0], read_len[1],
                    err[0], err[1], int(revcomp_odd), int(log_uniform_len))
     h = lib().spoa_synth_new(ctypes.byref(sp), seed, n_windows)

@@ -116,132 +116,186 @@ void ThreadPool::For(std::size_t n, const std::function<void(std::size_t)>& fn) 
 
 // --------------------------------------------------------------- drivers --
 
-BatchContext::BatchContext(const BatchOptions& opt)
-    : opt_(opt),
-      engine_(spoa::AlignmentEngine::Create(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c)),
-      pool_(opt.threads) {}
+BatchContext::BatchContext(const BatchOptions& opt) : opt_(opt), pool_(opt.threads) {
+  for (auto& e : engines_) e = spoa::AlignmentEngine::Create(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c);
+}
 
 std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats) {
   BatchContext ctx(opt);
   return RunBatched(ws, ctx, stats);
 }
 
+namespace {
+
+// One half of the windows and its engine: the jobs of the current round
+// and where their results go.
+struct Half {
+  spoa_hip_engine* dev = nullptr;
+  std::vector<std::uint32_t> windows;  // window ids of this half
+  std::vector<std::uint32_t> live;     // windows with a read in the current round
+  std::vector<spoa_hip_job> jobs;
+  std::vector<std::int32_t> job_of;    // per (live index, strand): job index or -1
+  std::vector<std::vector<std::int32_t>> pair_buf;
+  std::uint32_t round = 0;
+  bool pending = false;
+};
+
+}  // namespace
+
 std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, BatchStats* stats) {
   const auto t0 = Clock::now();
   BatchStats st;
   const BatchOptions& opt = ctx.options();
-  auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(ctx.engine());
-  if (!hip) throw std::invalid_argument("[spoa::hip] the alignment engine is not the HIP engine");
-  spoa_hip_engine* dev = hip->device();
   ThreadPool& pool = ctx.pool();
-
   const std::uint32_t W = ws.n_windows;
   std::vector<spoa::Graph> graphs(W);
   std::vector<WindowResult> out(W);
-  std::uint32_t max_reads = 0;
-  for (std::uint32_t w = 0; w < W; ++w) {
-    max_reads = std::max(max_reads, ws.win_read_off[w + 1] - ws.win_read_off[w]);
-  }
-  const int per_read = opt.strand ? 2 : 1;
   std::vector<std::string> rc_seq(W);
-  std::vector<std::uint32_t> live;
-  std::vector<spoa_hip_job> jobs;
-  std::vector<std::int32_t> job_of(W * static_cast<std::size_t>(per_read), -1);
-  std::vector<std::vector<std::int32_t>> pair_buf(W * static_cast<std::size_t>(per_read));
+  const int per_read = opt.strand ? 2 : 1;
 
-  for (std::uint32_t r = 0; r < max_reads; ++r) {
-    live.clear();
+  // halves balanced by total read bases, largest windows first
+  Half half[2];
+  for (int h = 0; h < 2; ++h) {
+    auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(ctx.engine(h));
+    if (!hip) throw std::invalid_argument("[spoa::hip] the alignment engine is not the HIP engine");
+    half[h].dev = hip->device();
+  }
+  std::uint32_t max_reads = 0;
+  {
+    std::vector<std::pair<std::uint64_t, std::uint32_t>> size(W);
     for (std::uint32_t w = 0; w < W; ++w) {
-      if (ws.win_read_off[w] + r < ws.win_read_off[w + 1]) live.push_back(w);
+      const std::uint32_t a = ws.win_read_off[w], b = ws.win_read_off[w + 1];
+      max_reads = std::max(max_reads, b - a);
+      size[w] = {ws.read_off[b] - ws.read_off[a], w};
     }
-    // build the jobs of this round (reverse complements in parallel)
+    std::sort(size.begin(), size.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    std::uint64_t load[2] = {0, 0};
+    for (const auto& sw : size) {
+      const int h = load[0] <= load[1] ? 0 : 1;
+      half[h].windows.push_back(sw.second);
+      load[h] += sw.first;
+    }
+    for (auto& hf : half) std::sort(hf.windows.begin(), hf.windows.end());
+  }
+
+  auto read_of = [&](std::uint32_t w, std::uint32_t r, std::uint32_t* len) {
+    const std::uint64_t k = ws.win_read_off[w] + r;
+    *len = static_cast<std::uint32_t>(ws.read_off[k + 1] - ws.read_off[k]);
+    return ws.seqs + ws.read_off[k];
+  };
+
+  // jobs of round r for half h (its graphs hold rounds < r)
+  auto build = [&](Half& hf, std::uint32_t r) {
+    hf.round = r;
+    hf.live.clear();
+    for (std::uint32_t w : hf.windows) {
+      if (ws.win_read_off[w] + r < ws.win_read_off[w + 1]) hf.live.push_back(w);
+    }
     if (opt.strand) {
-      pool.For(live.size(), [&](std::size_t x) {
-        const std::uint32_t w = live[x];
-        const std::uint64_t k = ws.win_read_off[w] + r;
-        rc_seq[w] = ReverseComplement(ws.seqs + ws.read_off[k], ws.read_off[k + 1] - ws.read_off[k]);
+      pool.For(hf.live.size(), [&](std::size_t x) {
+        const std::uint32_t w = hf.live[x];
+        std::uint32_t len;
+        const char* s = read_of(w, r, &len);
+        rc_seq[w] = ReverseComplement(s, len);
       });
     }
-    jobs.clear();
-    for (std::uint32_t w : live) {
-      const std::uint64_t k = ws.win_read_off[w] + r;
-      const char* s = ws.seqs + ws.read_off[k];
-      const std::uint32_t len = static_cast<std::uint32_t>(ws.read_off[k + 1] - ws.read_off[k]);
+    hf.jobs.clear();
+    hf.job_of.assign(hf.live.size() * per_read, -1);
+    if (hf.pair_buf.size() < hf.live.size() * per_read) hf.pair_buf.resize(hf.live.size() * per_read);
+    for (std::size_t x = 0; x < hf.live.size(); ++x) {
+      const std::uint32_t w = hf.live[x];
+      std::uint32_t len;
+      const char* s = read_of(w, r, &len);
+      if (graphs[w].nodes().empty() || len == 0) continue;
       for (int d = 0; d < per_read; ++d) {
-        const std::size_t slot = static_cast<std::size_t>(w) * per_read + d;
-        job_of[slot] = -1;
-        if (graphs[w].nodes().empty() || len == 0) continue;
         spoa_hip_job j{};
         j.graph = spoa::HipAlignmentEngine::View(graphs[w]);
         j.seq = d == 0 ? s : rc_seq[w].data();
         j.len = len;
         const std::uint32_t cap = j.graph.num_nodes + len;
-        pair_buf[slot].resize(2ull * cap);
-        j.pairs = pair_buf[slot].data();
+        auto& buf = hf.pair_buf[x * per_read + d];
+        buf.resize(2ull * cap);
+        j.pairs = buf.data();
         j.cap = cap;
-        job_of[slot] = static_cast<std::int32_t>(jobs.size());
-        jobs.push_back(j);
+        hf.job_of[x * per_read + d] = static_cast<std::int32_t>(hf.jobs.size());
+        hf.jobs.push_back(j);
       }
     }
-    if (!jobs.empty()) {
-      const auto ta = Clock::now();
-      const int rc = spoa_hip_align_batch(dev, jobs.data(), static_cast<std::uint32_t>(jobs.size()));
-      st.align_ms += Ms(ta, Clock::now());
-      if (rc != SPOA_HIP_OK) throw std::invalid_argument(spoa_hip_last_error());
-      spoa_hip_stats hs{};
-      spoa_hip_last_stats(dev, &hs);
-      st.kernel_ms += hs.kernel_ms;
-      st.h2d_ms += hs.h2d_ms;
-      st.d2h_ms += hs.d2h_ms;
-      st.pack_ms += hs.host_ms;
-      st.alloc_ms += hs.alloc_ms;
-      st.wait_ms += hs.wait_ms;
-      st.unpack_ms += hs.unpack_ms;
-      st.cells += hs.cells;
-      st.dp_bytes += hs.dp_bytes;
-      st.launches += hs.launches;
-      st.align_calls += jobs.size();
-    }
-    // merge the alignments (src/main.cpp:275-314 per window)
+  };
+
+  auto submit = [&](Half& hf) {
+    hf.pending = false;
+    if (hf.jobs.empty()) return;
+    const auto ta = Clock::now();
+    const int rc = spoa_hip_align_batch_submit(hf.dev, hf.jobs.data(), static_cast<std::uint32_t>(hf.jobs.size()));
+    st.align_ms += Ms(ta, Clock::now());
+    if (rc != SPOA_HIP_OK) throw std::invalid_argument(spoa_hip_last_error());
+    hf.pending = true;
+  };
+
+  auto wait = [&](Half& hf) {
+    if (!hf.pending) return;
+    hf.pending = false;
+    const auto ta = Clock::now();
+    const int rc = spoa_hip_align_batch_wait(hf.dev);
+    st.align_ms += Ms(ta, Clock::now());
+    if (rc != SPOA_HIP_OK) throw std::invalid_argument(spoa_hip_last_error());
+    spoa_hip_stats hs{};
+    spoa_hip_last_stats(hf.dev, &hs);
+    st.kernel_ms += hs.kernel_ms;
+    st.h2d_ms += hs.h2d_ms;
+    st.d2h_ms += hs.d2h_ms;
+    st.pack_ms += hs.host_ms;
+    st.alloc_ms += hs.alloc_ms;
+    st.wait_ms += hs.wait_ms;
+    st.unpack_ms += hs.unpack_ms;
+    st.cells += hs.cells;
+    st.dp_bytes += hs.dp_bytes;
+    st.launches += hs.launches;
+    st.align_calls += hf.jobs.size();
+  };
+
+  // merge the round's alignments into the half's graphs (src/main.cpp:275-314)
+  auto merge = [&](Half& hf) {
     const auto tg = Clock::now();
+    const std::uint32_t r = hf.round;
     std::mutex err_mu;
     std::string err;
-    pool.For(live.size(), [&](std::size_t x) {
+    pool.For(hf.live.size(), [&](std::size_t x) {
       try {
-      const std::uint32_t w = live[x];
-      const std::uint64_t k = ws.win_read_off[w] + r;
-      const char* s = ws.seqs + ws.read_off[k];
-      const std::uint32_t len = static_cast<std::uint32_t>(ws.read_off[k + 1] - ws.read_off[k]);
-      auto alignment_of = [&](int d, std::int32_t* score) {
-        spoa::Alignment a;
-        const std::int32_t ji = job_of[static_cast<std::size_t>(w) * per_read + d];
-        if (ji < 0) return a;
-        const spoa_hip_job& j = jobs[ji];
-        a.resize(j.n_pairs);
-        for (std::uint32_t p = 0; p < j.n_pairs; ++p) a[p] = {j.pairs[2 * p], j.pairs[2 * p + 1]};
-        if (j.has_score) *score = j.score;
-        return a;
-      };
-      std::int32_t score = 0, score_rev = 0;
-      spoa::Alignment a = alignment_of(0, &score);
-      bool reversed = false;
-      if (opt.strand) {
-        spoa::Alignment b = alignment_of(1, &score_rev);
-        if (!(score >= score_rev)) {
-          a.swap(b);
-          reversed = true;
+        const std::uint32_t w = hf.live[x];
+        std::uint32_t len;
+        const char* s = read_of(w, r, &len);
+        auto alignment_of = [&](int d, std::int32_t* score) {
+          spoa::Alignment a;
+          const std::int32_t ji = hf.job_of[x * per_read + d];
+          if (ji < 0) return a;
+          const spoa_hip_job& j = hf.jobs[ji];
+          a.resize(j.n_pairs);
+          for (std::uint32_t p = 0; p < j.n_pairs; ++p) a[p] = {j.pairs[2 * p], j.pairs[2 * p + 1]};
+          if (j.has_score) *score = j.score;
+          return a;
+        };
+        std::int32_t score = 0, score_rev = 0;
+        spoa::Alignment a = alignment_of(0, &score);
+        bool reversed = false;
+        if (opt.strand) {
+          spoa::Alignment b = alignment_of(1, &score_rev);
+          if (!(score >= score_rev)) {
+            a.swap(b);
+            reversed = true;
+          }
+          out[w].reversed.push_back(reversed ? 1 : 0);
         }
-        out[w].reversed.push_back(reversed ? 1 : 0);
-      }
-      const char* seq = reversed ? rc_seq[w].data() : s;
-      if (ws.quals) {
-        const char* q = ws.quals + ws.read_off[k];
-        std::vector<std::uint32_t> wt = Weights(q, len);
-        if (reversed) std::reverse(wt.begin(), wt.end());
-        graphs[w].AddAlignment(a, seq, len, wt);
-      } else {
-        graphs[w].AddAlignment(a, seq, len);
-      }
+        const char* seq = reversed ? rc_seq[w].data() : s;
+        if (ws.quals) {
+          const char* q = ws.quals + (s - ws.seqs);
+          std::vector<std::uint32_t> wt = Weights(q, len);
+          if (reversed) std::reverse(wt.begin(), wt.end());
+          graphs[w].AddAlignment(a, seq, len, wt);
+        } else {
+          graphs[w].AddAlignment(a, seq, len);
+        }
       } catch (const std::exception& ex) {
         std::lock_guard<std::mutex> lk(err_mu);
         if (err.empty()) err = ex.what();
@@ -249,8 +303,28 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
     });
     if (!err.empty()) throw std::invalid_argument(err);
     st.graph_ms += Ms(tg, Clock::now());
+  };
+
+  // Pipeline: the device aligns one half while the host merges the other.
+  for (std::uint32_t r = 0; r < max_reads; ++r) {
+    for (int h = 0; h < 2; ++h) {
+      Half& hf = half[h];
+      if (r > 0) {
+        wait(hf);
+        merge(hf);
+      }
+      build(hf, r);
+      submit(hf);
+    }
     ++st.rounds;
   }
+  for (int h = 0; h < 2; ++h) {
+    if (max_reads > 0) {
+      wait(half[h]);
+      merge(half[h]);
+    }
+  }
+
   const auto tf = Clock::now();
   pool.For(W, [&](std::size_t w) {
     out[w].consensus = graphs[w].GenerateConsensus();

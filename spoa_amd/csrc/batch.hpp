@@ -83,18 +83,20 @@ struct BatchStats {
   std::uint32_t rounds = 0;
 };
 
-// A long-lived engine and host thread pool for repeated batched runs with
-// one set of options (the engine keeps its device buffers between runs).
+// Long-lived engines and a host thread pool for repeated batched runs with
+// one set of options (the engines keep their device buffers between runs).
+// The windows are split in two halves with an engine each: while one half
+// aligns on the device, the host merges the other half's alignments.
 class BatchContext {
  public:
   explicit BatchContext(const BatchOptions& opt);
   const BatchOptions& options() const { return opt_; }
-  spoa::AlignmentEngine* engine() { return engine_.get(); }
+  spoa::AlignmentEngine* engine(int half = 0) { return engines_[half].get(); }
   ThreadPool& pool() { return pool_; }
 
  private:
   BatchOptions opt_;
-  std::unique_ptr<spoa::AlignmentEngine> engine_;
+  std::unique_ptr<spoa::AlignmentEngine> engines_[2];
   ThreadPool pool_;
 };
 

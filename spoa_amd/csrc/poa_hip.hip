@@ -256,6 +256,16 @@ void FirstColumns(const spoa_hip_params& p, const spoa_hip_graph& gr, int32_t* f
 
 }  // namespace
 
+struct Plan {
+  uint32_t job;
+  int width;
+  Cfg cfg;
+  uint32_t W, stride;
+  uint64_t mat_bytes;
+  uint64_t graph_words;
+  uint64_t seq_bytes;
+};
+
 constexpr int kStreams = 4;
 
 struct spoa_hip_engine {
@@ -272,6 +282,16 @@ struct spoa_hip_engine {
   // long-lived engine allocates once (allocation time grows with size)
   std::vector<DevBuf> arena;
   uint64_t arena_bytes = 0;
+  // a launched pass not yet finished (the asynchronous batch form)
+  struct Pending {
+    bool active = false;
+    spoa_hip_job* jobs = nullptr;
+    std::vector<Plan> plans;
+    size_t lo = 0, hi = 0;
+    uint64_t off_res = 0, off_pairs = 0;
+    std::vector<uint64_t> pair_at;
+    int first = 0;  // status of the jobs rejected before any launch
+  } pending;
   PinBuf h_in, h_out;
   spoa_hip_stats stats{};
   std::map<std::pair<KernelFn, int>, int> occupancy;
@@ -316,22 +336,13 @@ Cfg ChooseCfg(const spoa_hip_engine* E, uint32_t len, int width_bits, bool batch
   return DefaultCfg(E, W, width_bits, batch);
 }
 
-struct Plan {
-  uint32_t job;
-  int width;
-  Cfg cfg;
-  uint32_t W, stride;
-  uint64_t mat_bytes;
-  uint64_t graph_words;
-  uint64_t seq_bytes;
-};
-
 uint64_t MatBytes(const spoa_hip_params& p, uint32_t V, uint32_t stride, int width) {
   return RoundUp(static_cast<uint64_t>(V + 1) * Matrices(p.subtype) * stride * (width / 8), kAlign);
 }
 
 // Packs one job's graph block and read into the staging buffer.
-void PackJob(const spoa_hip_params& P, const spoa_hip_job& j, const Plan& pl, uint32_t* g, uint8_t* s) {
+void PackJob(const spoa_hip_params& P, const spoa_hip_job& j, const Plan& pl, uint32_t* g, uint8_t* s,
+             uint32_t* nslots_out, uint32_t* slot_chars_out) {
   const spoa_hip_graph& gr = j.graph;
   const uint32_t V = gr.num_nodes;
   uint32_t* rows = g;
@@ -343,12 +354,28 @@ void PackJob(const spoa_hip_params& P, const spoa_hip_job& j, const Plan& pl, ui
   uint32_t* preds = g + 9 * V + 4;
   const uint32_t base = gr.pred_off[0];
   FirstColumns(P, gr, fcH, fcF, fcO);
+  // profile slots: the distinct row bases, when there are at most 4
+  int16_t slot_of[256];
+  for (auto& x : slot_of) x = -1;
+  uint32_t nslots = 0, slot_chars = 0;
+  for (uint32_t r = 0; r < V && nslots <= 4; ++r) {
+    const uint8_t ch = gr.row_char[r];
+    if (slot_of[ch] < 0) {
+      if (nslots < 4) slot_chars |= static_cast<uint32_t>(ch) << (8 * nslots);
+      slot_of[ch] = static_cast<int16_t>(nslots++);
+    }
+  }
+  if (nslots > 4) nslots = 0;
+  *nslots_out = nslots;
+  *slot_chars_out = slot_chars;
   for (uint32_t r = 0; r < V; ++r) {
     const uint32_t pb = gr.pred_off[r] - base, pe = gr.pred_off[r + 1] - base;
     const uint32_t npred = pe > pb ? pe - pb : 1u;
+    const uint32_t slot = nslots ? static_cast<uint32_t>(slot_of[gr.row_char[r]]) : 7u;
     node_id[r] = gr.node_id[r];
     poff[r] = pb;
-    rows[4 * r + 0] = static_cast<uint32_t>(gr.row_char[r]) | (gr.is_sink[r] ? 0x100u : 0u) | (npred << 9);
+    rows[4 * r + 0] = static_cast<uint32_t>(gr.row_char[r]) | (gr.is_sink[r] ? 0x100u : 0u) | (slot << 9) |
+                      (npred << 12);
     rows[4 * r + 1] = pe > pb ? gr.pred_rank[gr.pred_off[r]] + 1 : 0u;
     rows[4 * r + 2] = pb;
     rows[4 * r + 3] = static_cast<uint32_t>(fcH[r + 1]);
@@ -372,7 +399,10 @@ bool SameGroup(const Plan& a, const Plan& b) {
 // transfer, aligned by one persistent launch per group -- the groups on
 // separate streams, so they share the device -- and copied back in one
 // transfer.
-int RunPass(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& plans, size_t lo, size_t hi) {
+int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& plans_in, size_t lo, size_t hi) {
+  auto& pd = E->pending;
+  pd.plans = std::move(plans_in);
+  const std::vector<Plan>& plans = pd.plans;
   const spoa_hip_params& P = E->prm;
   const uint32_t nj = static_cast<uint32_t>(hi - lo);
 
@@ -471,7 +501,8 @@ int RunPass(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pla
       d.stride = pl.stride;
       d.pair_off = static_cast<uint32_t>(p_at[x]);
       d.cap = j.graph.num_nodes + j.len;
-      PackJob(P, j, pl, E->h_in.At<uint32_t>(off_graph) + g_at[x], E->h_in.At<uint8_t>(off_seq) + s_at[x]);
+      PackJob(P, j, pl, E->h_in.At<uint32_t>(off_graph) + g_at[x], E->h_in.At<uint8_t>(off_seq) + s_at[x],
+              &d.nslots, &d.slot_chars);
     }
   };
   const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -492,6 +523,7 @@ int RunPass(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pla
     size_t a, b;  // job range inside the pass
     KernelFn fn;
     int threads;
+    uint32_t lds;  // query-profile bytes (4 bases x K/2 words a lane)
     uint32_t grid;
     KParams prm;
   };
@@ -504,12 +536,13 @@ int RunPass(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pla
     gp.b = y - lo;
     gp.fn = GroupKernel(E, plans[x]);
     gp.threads = 64 * plans[x].cfg.waves;
+    gp.lds = 8u * static_cast<uint32_t>(plans[x].cfg.k) * static_cast<uint32_t>(gp.threads);
     if (!gp.fn) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] unsupported launch configuration");
     int per_cu = 0;
     auto key = std::make_pair(gp.fn, gp.threads);
     auto oc = E->occupancy.find(key);
     if (oc == E->occupancy.end()) {
-      if (!HipOk(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gp.fn, gp.threads, 0), "occupancy")) {
+      if (!HipOk(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gp.fn, gp.threads, gp.lds), "occupancy")) {
         return SPOA_HIP_DEVICE_ERROR;
       }
       E->occupancy[key] = per_cu;
@@ -553,7 +586,7 @@ int RunPass(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pla
     args.seqs = din + off_seq;
     args.pairs = reinterpret_cast<int32_t*>(dout + off_pairs);
     args.results = reinterpret_cast<int32_t*>(dout + off_res) + 4 * gp.a;
-    gp.fn<<<gp.grid, gp.threads, 0, gs>>>(args);
+    gp.fn<<<gp.grid, gp.threads, gp.lds, gs>>>(args);
     if (!HipOk(hipGetLastError(), "kernel launch")) return SPOA_HIP_DEVICE_ERROR;
     E->stats.launches += 1;
   }
@@ -569,6 +602,25 @@ int RunPass(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pla
       !HipOk(hipEventRecord(E->ev3, st), "event")) {
     return SPOA_HIP_DEVICE_ERROR;
   }
+  pd.active = true;
+  pd.jobs = jobs;
+  pd.lo = lo;
+  pd.hi = hi;
+  pd.off_res = off_res;
+  pd.off_pairs = off_pairs;
+  pd.pair_at = std::move(p_at);
+  return SPOA_HIP_OK;
+}
+
+// Waits for the pending pass and unpacks its results into the job records.
+int RunPassFinish(spoa_hip_engine* E) {
+  auto& pd = E->pending;
+  if (!pd.active) return SPOA_HIP_OK;
+  pd.active = false;
+  spoa_hip_job* jobs = pd.jobs;
+  const std::vector<Plan>& plans = pd.plans;
+  const size_t lo = pd.lo, hi = pd.hi;
+  hipStream_t st = E->stream;
   const auto t_wait0 = std::chrono::steady_clock::now();
   if (!HipOk(hipStreamSynchronize(st), "kernel")) return SPOA_HIP_DEVICE_ERROR;
   const auto t_wait1 = std::chrono::steady_clock::now();
@@ -582,12 +634,11 @@ int RunPass(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pla
   E->stats.d2h_ms += ms_out;
 
   // ---- unpack: pairs come back in traceback order
-  const int32_t* res = E->h_out.At<int32_t>(off_res);
-  const int32_t* allp = E->h_out.At<int32_t>(off_pairs);
+  const int32_t* res = E->h_out.At<int32_t>(pd.off_res);
+  const int32_t* allp = E->h_out.At<int32_t>(pd.off_pairs);
   int first = SPOA_HIP_OK;
   for (size_t x = lo; x < hi; ++x) {
     spoa_hip_job& j = jobs[plans[x].job];
-    const DevJob& d = dj[x - lo];
     const int32_t* r4 = res + 4 * (x - lo);
     j.width_bits = plans[x].width;
     j.has_score = r4[2];
@@ -598,7 +649,7 @@ int RunPass(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pla
                      : (r4[3] == spoa_hip_dev::kDevInconsistent ? SPOA_HIP_INCONSISTENT : SPOA_HIP_OK);
     if (status == SPOA_HIP_OK && np > j.cap) status = SPOA_HIP_CAPACITY;
     if (status == SPOA_HIP_OK) {
-      const int32_t* src = allp + 2ull * d.pair_off;
+      const int32_t* src = allp + 2ull * pd.pair_at[x - lo];
       for (uint32_t k = 0; k < np; ++k) {
         j.pairs[2 * k] = src[2 * (np - 1 - k)];
         j.pairs[2 * k + 1] = src[2 * (np - 1 - k) + 1];
@@ -618,8 +669,13 @@ int RunPass(spoa_hip_engine* E, spoa_hip_job* jobs, const std::vector<Plan>& pla
   return first;
 }
 
-int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch) {
+int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch, bool async) {
+  if (E->pending.active) {
+    const int rc = RunPassFinish(E);
+    (void)rc;
+  }
   E->stats = spoa_hip_stats{};
+  E->pending.first = SPOA_HIP_OK;
   const spoa_hip_params& P = E->prm;
   int first = SPOA_HIP_OK;
   auto fail = [&](spoa_hip_job& j, int code, const char* msg) {
@@ -692,16 +748,21 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
       bytes += plans[hi].mat_bytes;
       ++hi;
     }
-    const int rc = RunPass(E, jobs, plans, lo, hi);
+    const bool last = hi == plans.size();
+    // the pass owns a copy of the plans while it is in flight
+    int rc = RunPassLaunch(E, jobs, std::vector<Plan>(plans), lo, hi);
+    if (rc == SPOA_HIP_OK && !(last && async)) rc = RunPassFinish(E);
     if (rc != SPOA_HIP_OK && first == SPOA_HIP_OK) first = rc;
-    if (rc == SPOA_HIP_DEVICE_ERROR || rc == SPOA_HIP_NO_MEMORY) {
+    if (rc == SPOA_HIP_DEVICE_ERROR || rc == SPOA_HIP_NO_MEMORY || rc == SPOA_HIP_INVALID_ARGUMENT) {
       const std::string msg = g_last_error;
+      E->pending.active = false;
       for (size_t x = lo; x < plans.size(); ++x) jobs[plans[x].job].status = rc;
       g_last_error = msg;
       return rc;
     }
     lo = hi;
   }
+  E->pending.first = first;
   return first;
 }
 
@@ -825,7 +886,7 @@ int spoa_hip_align(spoa_hip_engine* E, const spoa_hip_graph* graph, const char* 
   j.len = len;
   j.pairs = pairs;
   j.cap = cap;
-  const int rc = RunBatch(E, &j, 1, false);
+  const int rc = RunBatch(E, &j, 1, false, false);
   *n_pairs = j.n_pairs;
   if (has_score) *has_score = j.has_score;
   if (score && j.has_score) *score = j.score;
@@ -835,7 +896,24 @@ int spoa_hip_align(spoa_hip_engine* E, const spoa_hip_graph* graph, const char* 
 int spoa_hip_align_batch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs) {
   if (!E || (!jobs && n_jobs)) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null argument");
   if (!HipOk(hipSetDevice(E->device), "hipSetDevice")) return SPOA_HIP_DEVICE_ERROR;
-  return RunBatch(E, jobs, n_jobs, true);
+  return RunBatch(E, jobs, n_jobs, true, false);
+}
+
+int spoa_hip_align_batch_submit(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs) {
+  if (!E || (!jobs && n_jobs)) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null argument");
+  if (!HipOk(hipSetDevice(E->device), "hipSetDevice")) return SPOA_HIP_DEVICE_ERROR;
+  const int rc = RunBatch(E, jobs, n_jobs, true, true);
+  return rc == SPOA_HIP_DEVICE_ERROR || rc == SPOA_HIP_NO_MEMORY ? rc : SPOA_HIP_OK;
+}
+
+int spoa_hip_align_batch_wait(spoa_hip_engine* E) {
+  if (!E) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null engine");
+  if (!HipOk(hipSetDevice(E->device), "hipSetDevice")) return SPOA_HIP_DEVICE_ERROR;
+  const std::string before = g_last_error;
+  const int rc = RunPassFinish(E);
+  if (rc != SPOA_HIP_OK) return rc;
+  if (E->pending.first != SPOA_HIP_OK) g_last_error = before;
+  return E->pending.first;
 }
 
 int spoa_hip_last_stats(spoa_hip_engine* E, spoa_hip_stats* out) {

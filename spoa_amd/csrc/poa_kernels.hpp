@@ -62,12 +62,15 @@ struct DevJob {
   uint32_t stride;     // row stride of one matrix, a multiple of K
   uint32_t pair_off;   // pairs into the pair buffer
   uint32_t cap;        // pair capacity (V + n suffices)
+  uint32_t nslots;     // distinct row bases with an LDS profile (0: formed per row)
+  uint32_t slot_chars; // the base of each profile slot, a byte each
 };
 
 // Graph block of a job (32-bit words from graph_off, 16-byte aligned):
-//   rows[V] (uint4 per rank: decoded base | sink << 8 | predecessor count
-//   << 9 (1 for a source, whose predecessor is row 0), first predecessor
-//   row, offset of the predecessor list in preds, fcH of the row),
+//   rows[V] (uint4 per rank: decoded base | sink << 8 | profile slot << 9
+//   (7: none) | predecessor count << 12 (1 for a source, whose predecessor
+//   is row 0), first predecessor row, offset of the predecessor list in
+//   preds, fcH of the row),
 //   node_id[V], pred_off[V + 1] (relative to preds), fcH[V + 1],
 //   fcF[V + 1], fcO[V + 1], preds[...] (predecessor rows, rank + 1, in
 //   Node::inedges order).
@@ -107,58 +110,6 @@ struct KArgs {
 enum DevStatus : int32_t { kDevOk = 0, kDevCapacity = 6, kDevInconsistent = 7 };
 
 // ------------------------------------------------------------- helpers --
-
-template <typename ST, int K>
-__device__ __forceinline__ void LoadCols(const ST* __restrict__ p, int32_t (&v)[K]) {
-  if constexpr (sizeof(ST) == 2) {
-    static_assert(K % 8 == 0, "int16 rows move 8 columns per 16-byte access");
-    const uint4* q = reinterpret_cast<const uint4*>(p);
-#pragma unroll
-    for (int i = 0; i < K / 8; ++i) {
-      const uint4 u = q[i];
-      const uint32_t w[4] = {u.x, u.y, u.z, u.w};
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        v[i * 8 + 2 * j] = static_cast<int32_t>(static_cast<int16_t>(w[j] & 0xffffu));
-        v[i * 8 + 2 * j + 1] = static_cast<int32_t>(w[j]) >> 16;
-      }
-    }
-  } else {
-    static_assert(K % 4 == 0, "int32 rows move 4 columns per 16-byte access");
-    const int4* q = reinterpret_cast<const int4*>(p);
-#pragma unroll
-    for (int i = 0; i < K / 4; ++i) {
-      const int4 u = q[i];
-      v[i * 4 + 0] = u.x;
-      v[i * 4 + 1] = u.y;
-      v[i * 4 + 2] = u.z;
-      v[i * 4 + 3] = u.w;
-    }
-  }
-}
-
-template <typename ST, int K>
-__device__ __forceinline__ void StoreCols(ST* __restrict__ p, const int32_t (&v)[K]) {
-  if constexpr (sizeof(ST) == 2) {
-    uint4* q = reinterpret_cast<uint4*>(p);
-#pragma unroll
-    for (int i = 0; i < K / 8; ++i) {
-      uint32_t w[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        w[j] = (static_cast<uint32_t>(v[i * 8 + 2 * j]) & 0xffffu) |
-               (static_cast<uint32_t>(v[i * 8 + 2 * j + 1]) << 16);
-      }
-      q[i] = make_uint4(w[0], w[1], w[2], w[3]);
-    }
-  } else {
-    int4* q = reinterpret_cast<int4*>(p);
-#pragma unroll
-    for (int i = 0; i < K / 4; ++i) {
-      q[i] = make_int4(v[i * 4 + 0], v[i * 4 + 1], v[i * 4 + 2], v[i * 4 + 3]);
-    }
-  }
-}
 
 // y <- M (x) y for a 2x2 max-plus matrix (row-major).
 __device__ __forceinline__ void MpApply(const int32_t* M, int32_t& y0, int32_t& y1) {
@@ -384,86 +335,157 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
   *status_out = status;
 }
 
-// ------------------------------------------------------- packed rows --
-// A lane's K columns of one stored row as the 32-bit words the 16-byte
-// loads and stores move (int16: two columns a word).  The previous row is
-// kept in this form in registers, bit-identical to what memory holds.
-template <typename ST, int K>
-constexpr int Words() {
-  return sizeof(ST) == 2 ? K / 2 : K;
-}
+// ------------------------------------------------------------ word ops --
+// A lane's K columns of a score row live in 32-bit words: two int16
+// columns a word (column 2i in the low half) or one int32 column a word.
+// The int16 words are worked on with packed 16-bit instructions, whose
+// wrap-around is the reference's int16 SIMD arithmetic.
+typedef short S16x2 __attribute__((ext_vector_type(2)));
 
-template <typename ST, int K>
-__device__ __forceinline__ void Pack(const int32_t (&v)[K], uint32_t (&w)[Words<ST, K>()]) {
+template <typename ST>
+struct Cols {
+  static constexpr int kPerWord = sizeof(ST) == 2 ? 2 : 1;
+};
+
+template <typename ST>
+__device__ __forceinline__ uint32_t Splat(int32_t v) {
   if constexpr (sizeof(ST) == 2) {
-#pragma unroll
-    for (int i = 0; i < K / 2; ++i) {
-      w[i] = (static_cast<uint32_t>(v[2 * i]) & 0xffffu) | (static_cast<uint32_t>(v[2 * i + 1]) << 16);
-    }
+    return (static_cast<uint32_t>(v) & 0xffffu) * 0x10001u;
   } else {
-#pragma unroll
-    for (int i = 0; i < K; ++i) w[i] = static_cast<uint32_t>(v[i]);
+    return static_cast<uint32_t>(v);
   }
 }
 
-template <typename ST, int K>
-__device__ __forceinline__ void Unpack(const uint32_t (&w)[Words<ST, K>()], int32_t (&v)[K]) {
+// a + b per column (b already splat)
+template <typename ST>
+__device__ __forceinline__ uint32_t WAdd(uint32_t a, uint32_t b) {
   if constexpr (sizeof(ST) == 2) {
-#pragma unroll
-    for (int i = 0; i < K / 2; ++i) {
-      v[2 * i] = static_cast<int32_t>(static_cast<int16_t>(w[i] & 0xffffu));
-      v[2 * i + 1] = static_cast<int32_t>(w[i]) >> 16;
-    }
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(S16x2, a) + __builtin_bit_cast(S16x2, b));
   } else {
-#pragma unroll
-    for (int i = 0; i < K; ++i) v[i] = static_cast<int32_t>(w[i]);
+    return a + b;
   }
 }
 
-template <typename ST, int K>
-__device__ __forceinline__ void LoadWords(const ST* __restrict__ p, uint32_t (&w)[Words<ST, K>()]) {
-  const uint4* q = reinterpret_cast<const uint4*>(p);
-#pragma unroll
-  for (int i = 0; i < Words<ST, K>() / 4; ++i) {
-    const uint4 u = q[i];
-    w[4 * i + 0] = u.x;
-    w[4 * i + 1] = u.y;
-    w[4 * i + 2] = u.z;
-    w[4 * i + 3] = u.w;
+template <typename ST>
+__device__ __forceinline__ uint32_t WMax(uint32_t a, uint32_t b) {
+  if constexpr (sizeof(ST) == 2) {
+    return __builtin_bit_cast(
+        uint32_t, __builtin_elementwise_max(__builtin_bit_cast(S16x2, a), __builtin_bit_cast(S16x2, b)));
+  } else {
+    return static_cast<uint32_t>(max(static_cast<int32_t>(a), static_cast<int32_t>(b)));
   }
 }
 
-template <typename ST, int K>
-__device__ __forceinline__ void StoreWords(ST* __restrict__ p, const uint32_t (&w)[Words<ST, K>()]) {
-  uint4* q = reinterpret_cast<uint4*>(p);
+// column k (of the word array) as a 32-bit score
+template <typename ST, int KW>
+__device__ __forceinline__ int32_t Col(const uint32_t (&w)[KW], int k) {
+  if constexpr (sizeof(ST) == 2) {
+    return (k & 1) ? static_cast<int32_t>(w[k >> 1]) >> 16 : static_cast<int32_t>(static_cast<int16_t>(w[k >> 1]));
+  } else {
+    return static_cast<int32_t>(w[k]);
+  }
+}
+
+// the last column of a word array
+template <typename ST, int KW>
+__device__ __forceinline__ int32_t LastCol(const uint32_t (&w)[KW]) {
+  if constexpr (sizeof(ST) == 2) {
+    return static_cast<int32_t>(w[KW - 1]) >> 16;
+  } else {
+    return static_cast<int32_t>(w[KW - 1]);
+  }
+}
+
+// Buffer resource over one DP row (all its matrices); out-of-range
+// accesses read 0 and drop stores.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t RowRsrc(const void* row, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), 0, static_cast<int>(bytes), 0x00020000);
+}
+
+template <int KW>
+__device__ __forceinline__ void BufLoad(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t (&w)[KW]) {
+  static_assert(KW % 4 == 0, "rows move in 16-byte accesses");
 #pragma unroll
-  for (int i = 0; i < Words<ST, K>() / 4; ++i) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+  for (int i = 0; i < KW / 4; ++i) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u * i, 0, 0);
+    w[4 * i + 0] = v[0];
+    w[4 * i + 1] = v[1];
+    w[4 * i + 2] = v[2];
+    w[4 * i + 3] = v[3];
+  }
+}
+
+template <int KW>
+__device__ __forceinline__ void BufStore(__amdgpu_buffer_rsrc_t r, uint32_t off, const uint32_t (&w)[KW]) {
+#pragma unroll
+  for (int i = 0; i < KW / 4; ++i) {
+    typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+    U4 v;
+    v[0] = w[4 * i + 0];
+    v[1] = w[4 * i + 1];
+    v[2] = w[4 * i + 2];
+    v[3] = w[4 * i + 3];
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, off + 16u * i, 0, 0);
+  }
+}
+
+template <typename ST>
+__device__ __forceinline__ int32_t BufLoadCol(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(ST) == 2) {
+    return static_cast<int32_t>(static_cast<int16_t>(__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0)));
+  } else {
+    return static_cast<int32_t>(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  }
+}
+
+// Packed int16 profile words (2 columns a word) of a lane's K columns for
+// the row base rch: m on a match, n on a mismatch, pad past the read.
+template <int K>
+__device__ __forceinline__ void ProfileWords(const uint32_t (&scp)[K / 4], int32_t nvalid, uint32_t rch, int32_t m,
+                                             int32_t mis, int32_t pad, uint32_t (&pw)[K / 2]) {
+#pragma unroll
+  for (int i = 0; i < K / 2; ++i) {
+    const uint32_t c0 = (scp[i / 2] >> (16 * (i % 2))) & 0xffu;
+    const uint32_t c1 = (scp[i / 2] >> (16 * (i % 2) + 8)) & 0xffu;
+    const int32_t p0 = 2 * i < nvalid ? (c0 == rch ? m : mis) : pad;
+    const int32_t p1 = 2 * i + 1 < nvalid ? (c1 == rch ? m : mis) : pad;
+    pw[i] = (static_cast<uint32_t>(p0) & 0xffffu) | (static_cast<uint32_t>(p1) << 16);
+  }
 }
 
 // --------------------------------------------------------------- kernel --
 // SUB: 0 linear, 1 affine, 2 convex.  ST: stored score type.  K: columns
 // per lane.  MAXT: launch bound (threads per workgroup).
+//
+// Dynamic LDS: the job's query profile, 4 row bases x K/2 words per lane
+// (8 * K * blockDim.x bytes), used when a row is one tile and the graph
+// has at most 4 distinct bases (the profile is otherwise formed per row).
 template <int SUB, typename ST, int K, int MAXT>
 __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
   constexpr int L = sizeof(ST) == 2 ? 16 : 8;
-  constexpr int NM = SUB == 0 ? 1 : (SUB == 1 ? 3 : 5);  // H | F E | O Q
-  constexpr int KW = Words<ST, K>();
+  constexpr int CPW = Cols<ST>::kPerWord;
+  constexpr int KW = K / CPW;  // score words a lane
+  constexpr int PW = K / 2;    // profile words a lane
+  constexpr int S = sizeof(ST);
   static_assert(SUB != 2 || K % L == 0, "convex lanes must start reference segments");
-  static_assert(KW % 4 == 0, "rows move in 16-byte accesses");
+  static_assert(KW % 4 == 0 && PW % 4 == 0, "rows move in 16-byte accesses");
   constexpr int32_t kMinI = -2147483647 - 1;
 
   __shared__ int32_t s_tot[2][kMaxWaves][2];
   __shared__ int32_t s_red[kMaxWaves][3];
   __shared__ uint32_t s_job;
+  extern __shared__ uint32_t s_prof[];
 
   const KParams& prm = a.prm;
   const int tid = threadIdx.x;
+  const int nthreads = blockDim.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int nwaves = static_cast<int>(blockDim.x) >> 6;
+  const int nwaves = nthreads >> 6;
   const int type = prm.type;
   const int32_t m = prm.m, mis = prm.mis, g = prm.g, e = prm.e, q = prm.q, cc = prm.c;
   const int32_t pad = prm.pad;
+  const uint32_t G2 = Splat<ST>(g), E2 = Splat<ST>(e), Q2 = Splat<ST>(q), C2 = Splat<ST>(cc);
 
   for (;;) {
     if (tid == 0) s_job = atomicAdd(a.queue, 1u);
@@ -477,11 +499,15 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
     const GraphView gv = MakeView(a.graphs + J.graph_off, V);
     const uint8_t* __restrict__ seq = a.seqs + J.seq_off;
     ST* __restrict__ mat = reinterpret_cast<ST*>(J.mat_addr);
-    const uint64_t rstride = static_cast<uint64_t>(stride) * NM;
-    const uint32_t tile_w = static_cast<uint32_t>(nwaves) * 64u * K;
+    const uint32_t NM = SUB == 0 ? 1u : (SUB == 1 ? 3u : 5u);
+    const uint32_t rbytes = stride * NM * S;  // one DP row, all matrices
+    const uint32_t mbytes = stride * S;       // one matrix of a row
+    const uint32_t tile_w = static_cast<uint32_t>(nthreads) * K;
     const uint32_t ntiles = (stride + tile_w - 1) / tile_w;
     const bool single = ntiles == 1;  // a row is one tile: the previous row stays in registers
     const bool sync = nwaves > 1 || ntiles > 1;
+    const uint32_t nslots = single ? Uniform(J.nslots) : 0u;
+    auto row_ptr = [&](uint32_t r) -> ST* { return mat + static_cast<uint64_t>(r) * stride * NM; };
 
     // row 0 (:556-694): E = g + c*e, Q = q + c*c, F = O = -inf, H per type
     auto row0_h = [&](int32_t ci) -> int32_t {
@@ -490,7 +516,7 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
       if (SUB == 1) return g + ci * e;
       return max(g + ci * e, q + ci * cc);
     };
-    for (uint32_t col = tid; col < stride; col += blockDim.x) {
+    for (uint32_t col = tid; col < stride; col += nthreads) {
       const int32_t ci = static_cast<int32_t>(col);
       mat[col] = static_cast<ST>(row0_h(ci));
       if (SUB >= 1) {
@@ -502,6 +528,35 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
         mat[4 * stride + col] = static_cast<ST>(q + ci * cc);
       }
     }
+
+    // the read's bases under this lane's columns (reloaded per tile when a
+    // row spans several tiles) and, with few distinct row bases, the
+    // lane's query profile in LDS (written and read by this lane only)
+    uint32_t scp[K / 4];
+    int32_t nvalid;
+    LoadChars<K>(seq, static_cast<uint32_t>(tid) * K, n, static_cast<uint32_t>(tid) * K < stride, scp, nvalid);
+    for (uint32_t sl = 0; sl < nslots; ++sl) {
+      uint32_t pw[PW];
+      ProfileWords<K>(scp, nvalid, (J.slot_chars >> (8 * sl)) & 0xffu, m, mis, pad, pw);
+#pragma unroll
+      for (int i = 0; i < PW; ++i) s_prof[(sl * PW + i) * nthreads + tid] = pw[i];
+    }
+
+    // the previous row (H, F, O words of this lane's columns), from row 0
+    uint32_t cH[KW], cF[KW], cO[KW];
+    if (single) {
+#pragma unroll
+      for (int i = 0; i < KW; ++i) {
+        const int32_t c = tid * K + i * CPW;
+        if constexpr (CPW == 2) {
+          cH[i] = (static_cast<uint32_t>(row0_h(c)) & 0xffffu) | (static_cast<uint32_t>(row0_h(c + 1)) << 16);
+        } else {
+          cH[i] = static_cast<uint32_t>(row0_h(c));
+        }
+        cF[i] = Splat<ST>(prm.ninf);
+        cO[i] = Splat<ST>(prm.ninf);
+      }
+    }
     __syncthreads();
 
     // End-cell candidates per lane: the lane's best (floored) value and the
@@ -509,38 +564,23 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
     int32_t best = type == 0 ? 0 : prm.ninf;
     int32_t best_row = -1;
     int par = 0;
-    // the read's bases under this lane's columns (reloaded per tile when a
-    // row spans several tiles)
-    uint32_t scp[K / 4];
-    int32_t nvalid;
-    LoadChars<K>(seq, static_cast<uint32_t>(tid) * K, n, static_cast<uint32_t>(tid) * K < stride, scp, nvalid);
-    // the previous row (H, F, O) of this lane's columns, starting with row 0
-    uint32_t cH[KW], cF[KW], cO[KW];
-    if (single) {
-      int32_t v[K], f[K];
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        v[k] = row0_h(tid * K + k);
-        f[k] = prm.ninf;
-      }
-      Pack<ST, K>(v, cH);
-      Pack<ST, K>(f, cF);
-      Pack<ST, K>(f, cO);
-    }
     int32_t fch_prev = gv.fcH[0];
     uint4 rec = gv.rows[0];
 
     for (uint32_t r = 1; r <= V; ++r) {
-      // row record: base | sink | predecessor count, first predecessor row,
-      // offset of the predecessor list, first-column score (next one prefetched)
+      // row record: base | sink << 8 | profile slot << 9 | predecessor count
+      // << 12, first predecessor row, offset of the predecessor list,
+      // first-column score (the next record is prefetched)
       const uint4 cur = rec;
       if (r < V) rec = gv.rows[r];
       const uint32_t rch = cur.x & 0xffu;
       const bool sink = ((cur.x >> 8) & 1u) != 0;
-      const uint32_t npred = cur.x >> 9;
+      const uint32_t slot = (cur.x >> 9) & 7u;
+      const uint32_t npred = cur.x >> 12;
       const uint32_t p0 = cur.y, pb = cur.z;
       const int32_t fch_r = static_cast<int32_t>(cur.w);
-      ST* __restrict__ row = mat + static_cast<uint64_t>(r) * rstride;
+      ST* __restrict__ row = row_ptr(r);
+      const __amdgpu_buffer_rsrc_t rrow = RowRsrc(row, rbytes);
       // carry entering column 0: linear H, affine E = fc + g, convex (E, Q)
       int32_t y0 = fch_r + g;
       int32_t y1 = fch_r + q;
@@ -548,115 +588,145 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
       for (uint32_t t = 0; t < ntiles; ++t) {
         const uint32_t c0 = t * tile_w + static_cast<uint32_t>(tid) * K;
         const bool active = c0 < stride;
+        const uint32_t off = c0 * S;  // byte offset of the lane's columns in a matrix
+
+        // ---- query profile of the row base
+        if (!single) LoadChars<K>(seq, c0, n, active, scp, nvalid);
+        uint32_t pw[PW];
+        if (slot < nslots) {
+#pragma unroll
+          for (int i = 0; i < PW; ++i) pw[i] = s_prof[(slot * PW + i) * nthreads + tid];
+        } else {
+          ProfileWords<K>(scp, nvalid, rch, m, mis, pad, pw);
+        }
 
         // ---- 1. vertical pass over the predecessors, in in-edge order
-        if (!single) LoadChars<K>(seq, c0, n, active, scp, nvalid);
-        int32_t h1[K], fv[K], ov[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          h1[k] = kMinI;
-          fv[k] = kMinI;
-          ov[k] = kMinI;
-        }
+        uint32_t h1[KW], fv[KW], ov[KW];
         for (uint32_t pi = 0; pi < npred; ++pi) {
           const uint32_t p = pi == 0 ? p0 : Uniform(gv.preds[pb + pi]);
-          const ST* __restrict__ prow = mat + static_cast<uint64_t>(p) * rstride;
           const bool cached = single && p + 1 == r;
-          int32_t hv[K];
+          const __amdgpu_buffer_rsrc_t rp = RowRsrc(row_ptr(p), rbytes);
+          uint32_t hv[KW];
           if (cached) {
-            Unpack<ST, K>(cH, hv);
-          } else if (active) {
-            LoadCols<ST, K>(prow + c0, hv);
-          } else {
 #pragma unroll
-            for (int k = 0; k < K; ++k) hv[k] = 0;
+            for (int i = 0; i < KW; ++i) hv[i] = cH[i];
+          } else {
+            BufLoad<KW>(rp, active ? off : rbytes, hv);
           }
           // H_p[c0 - 1]: neighbouring lane, or the first column / memory
           // across waves and tiles
-          int32_t hm1 = __shfl_up(hv[K - 1], 1, 64);
+          int32_t hm1 = __shfl_up(LastCol<ST, KW>(hv), 1, 64);
           if (lane == 0) {
             if (c0 == 0) {
               hm1 = p + 1 == r ? fch_prev : gv.fcH[p];
             } else {
-              hm1 = active ? static_cast<int32_t>(prow[c0 - 1]) : 0;
+              hm1 = BufLoadCol<ST>(rp, active ? off - S : rbytes);
             }
           }
           // diagonal (:838-850): H_p[c - 1] + profile
-          {
-            int32_t prev = hm1;
+          uint32_t dg[KW];
+          if constexpr (CPW == 2) {
+            dg[0] = WAdd<ST>((hv[0] << 16) | (static_cast<uint32_t>(hm1) & 0xffffu), pw[0]);
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-              const uint32_t ch = (scp[k / 4] >> (8 * (k % 4))) & 0xffu;
-              const int32_t pr = k < nvalid ? (ch == rch ? m : mis) : pad;
-              h1[k] = max(h1[k], prev + pr);
-              prev = hv[k];
+            for (int i = 1; i < KW; ++i) dg[i] = WAdd<ST>(__builtin_amdgcn_alignbit(hv[i], hv[i - 1], 16), pw[i]);
+          } else {
+#pragma unroll
+            for (int i = 0; i < KW; ++i) {
+              const int32_t pr = (i & 1) ? static_cast<int32_t>(pw[i >> 1]) >> 16
+                                         : static_cast<int32_t>(static_cast<int16_t>(pw[i >> 1]));
+              dg[i] = static_cast<uint32_t>((i == 0 ? hm1 : static_cast<int32_t>(hv[i - 1])) + pr);
             }
           }
+          // vertical moves: linear H_p + g; affine / convex the gap rows
+          uint32_t vf[KW], vo[KW];
           if (SUB == 0) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) fv[k] = max(fv[k], hv[k] + g);
+            for (int i = 0; i < KW; ++i) vf[i] = WAdd<ST>(hv[i], G2);
           } else {
-            int32_t fp[K];
+            uint32_t fp[KW];
             if (cached) {
-              Unpack<ST, K>(cF, fp);
-            } else if (active) {
-              LoadCols<ST, K>(prow + stride + c0, fp);
-            } else {
 #pragma unroll
-              for (int k = 0; k < K; ++k) fp[k] = 0;
+              for (int i = 0; i < KW; ++i) fp[i] = cF[i];
+            } else {
+              BufLoad<KW>(rp, active ? off + mbytes : rbytes, fp);
             }
 #pragma unroll
-            for (int k = 0; k < K; ++k) fv[k] = max(fv[k], max(hv[k] + g, fp[k] + e));
+            for (int i = 0; i < KW; ++i) vf[i] = WMax<ST>(WAdd<ST>(hv[i], G2), WAdd<ST>(fp[i], E2));
             if (SUB == 2) {
-              int32_t op[K];
+              uint32_t op[KW];
               if (cached) {
-                Unpack<ST, K>(cO, op);
-              } else if (active) {
-                LoadCols<ST, K>(prow + 3 * stride + c0, op);
-              } else {
 #pragma unroll
-                for (int k = 0; k < K; ++k) op[k] = 0;
+                for (int i = 0; i < KW; ++i) op[i] = cO[i];
+              } else {
+                BufLoad<KW>(rp, active ? off + 3 * mbytes : rbytes, op);
               }
 #pragma unroll
-              for (int k = 0; k < K; ++k) ov[k] = max(ov[k], max(hv[k] + q, op[k] + cc));
+              for (int i = 0; i < KW; ++i) vo[i] = WMax<ST>(WAdd<ST>(hv[i], Q2), WAdd<ST>(op[i], C2));
+            }
+          }
+          if (pi == 0) {
+#pragma unroll
+            for (int i = 0; i < KW; ++i) {
+              h1[i] = dg[i];
+              fv[i] = vf[i];
+              if (SUB == 2) ov[i] = vo[i];
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < KW; ++i) {
+              h1[i] = WMax<ST>(h1[i], dg[i]);
+              fv[i] = WMax<ST>(fv[i], vf[i]);
+              if (SUB == 2) ov[i] = WMax<ST>(ov[i], vo[i]);
             }
           }
         }
-        // H1: the best of the vertical and diagonal moves
+        // H1: the best of the vertical and diagonal moves; F and O are final
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          h1[k] = max(h1[k], fv[k]);
-          if (SUB == 2) h1[k] = max(h1[k], ov[k]);
+        for (int i = 0; i < KW; ++i) {
+          h1[i] = WMax<ST>(h1[i], fv[i]);
+          if (SUB == 2) h1[i] = WMax<ST>(h1[i], ov[i]);
+        }
+        if (SUB >= 1 && active) {
+          BufStore<KW>(rrow, off + mbytes, fv);
+          if (SUB == 2) BufStore<KW>(rrow, off + 3 * mbytes, ov);
+        }
+        if (single) {
+#pragma unroll
+          for (int i = 0; i < KW; ++i) {
+            cF[i] = fv[i];
+            if (SUB == 2) cO[i] = ov[i];
+          }
         }
 
         // ---- 2. lane transfer with the carry absent
         int32_t b0 = 0, b1 = 0;
         if (SUB == 0) {
-          int32_t x = h1[0];
+          int32_t x = Col<ST, KW>(h1, 0);
 #pragma unroll
-          for (int k = 1; k < K; ++k) x = max(h1[k], x + g);
+          for (int k = 1; k < K; ++k) x = max(Col<ST, KW>(h1, k), x + g);
           b0 = x + g;
         } else if (SUB == 1) {
-          int32_t x = h1[0] + g;  // E at column c0 + 1
+          int32_t x = Col<ST, KW>(h1, 0) + g;  // E at column c0 + 1
 #pragma unroll
-          for (int k = 2; k < K; ++k) x = max(h1[k - 1] + g, x + e);
-          b0 = max(h1[K - 1] + g, x + e);
+          for (int k = 2; k < K; ++k) x = max(Col<ST, KW>(h1, k - 1) + g, x + e);
+          b0 = max(Col<ST, KW>(h1, K - 1) + g, x + e);
         } else {
-          int32_t xe = h1[0] + g, xq = h1[0] + q;  // E, Q at column c0 + 1
+          int32_t xe = Col<ST, KW>(h1, 0) + g, xq = Col<ST, KW>(h1, 0) + q;  // E, Q at column c0 + 1
 #pragma unroll
           for (int k = 2; k < K; ++k) {
+            const int32_t hp = Col<ST, KW>(h1, k - 1);
             if (k % L == 0) {
-              const int32_t hf = max(h1[k - 1], max(xe, xq));
+              const int32_t hf = max(hp, max(xe, xq));
               const int32_t ne = max(hf + g, xe + e);
               const int32_t nq = max(hf + q, xq + cc);
               xe = ne;
               xq = nq;
             } else {
-              xe = max(h1[k - 1] + g, xe + e);
-              xq = max(h1[k - 1] + q, xq + cc);
+              xe = max(hp + g, xe + e);
+              xq = max(hp + q, xq + cc);
             }
           }
-          const int32_t hf = max(h1[K - 1], max(xe, xq));
+          const int32_t hf = max(Col<ST, KW>(h1, K - 1), max(xe, xq));
           b0 = max(hf + g, xe + e);
           b1 = max(hf + q, xq + cc);
         }
@@ -741,74 +811,84 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
           x1 = lane == 0 ? w1 : max(e1, u1);
         }
 
-        // ---- 3. exact pass, stores, end-cell bookkeeping
-        if (active) {
-          int32_t hs[K];
-          if (SUB == 0) {
-            int32_t x = max(h1[0], x0);
-            hs[0] = type == 0 ? max(x, 0) : x;
+        // ---- 3. exact pass: E, Q, H column by column, packed as produced
+        uint32_t hw[KW];
+        {
+          uint32_t ew[KW], qw[KW];
+          int32_t pe = x0, pq = x1;  // E, Q at the previous column (before column c0: the carry)
+          int32_t hlo = 0, elo = 0, qlo = 0;
 #pragma unroll
-            for (int k = 1; k < K; ++k) {
-              x = max(h1[k], x + g);
-              hs[k] = type == 0 ? max(x, 0) : x;
-            }
-          } else {
-            int32_t ev[K], qv[K];
-            ev[0] = x0;
-            qv[0] = x1;
-#pragma unroll
-            for (int k = 1; k < K; ++k) {
-              if (SUB == 2 && k % L == 0) {
-                const int32_t hf = max(h1[k - 1], max(ev[k - 1], qv[k - 1]));
-                ev[k] = max(hf + g, ev[k - 1] + e);
-                qv[k] = max(hf + q, qv[k - 1] + cc);
+          for (int k = 0; k < K; ++k) {
+            const int32_t h1k = Col<ST, KW>(h1, k);
+            int32_t h;
+            int32_t ek = 0, qk = 0;
+            if (SUB == 0) {
+              h = k == 0 ? max(h1k, x0) : max(h1k, pe + g);
+              pe = h;
+            } else {
+              if (k == 0) {
+                ek = x0;
+                qk = x1;
               } else {
-                ev[k] = max(h1[k - 1] + g, ev[k - 1] + e);
-                if (SUB == 2) qv[k] = max(h1[k - 1] + q, qv[k - 1] + cc);
+                const int32_t hp = Col<ST, KW>(h1, k - 1);
+                if (SUB == 2 && k % L == 0) {
+                  const int32_t hf = max(hp, max(pe, pq));
+                  ek = max(hf + g, pe + e);
+                  qk = max(hf + q, pq + cc);
+                } else {
+                  ek = max(hp + g, pe + e);
+                  if (SUB == 2) qk = max(hp + q, pq + cc);
+                }
               }
+              h = max(h1k, ek);
+              if (SUB == 2) h = max(h, qk);
+              pe = ek;
+              pq = qk;
             }
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-              int32_t h = max(h1[k], ev[k]);
-              if (SUB == 2) h = max(h, qv[k]);
-              hs[k] = type == 0 ? max(h, 0) : h;
-            }
-            uint32_t wF[KW], wE[KW];
-            Pack<ST, K>(fv, wF);
-            Pack<ST, K>(ev, wE);
-            StoreWords<ST, K>(row + stride + c0, wF);
-            StoreWords<ST, K>(row + 2 * stride + c0, wE);
-            if (single) {
-#pragma unroll
-              for (int i = 0; i < KW; ++i) cF[i] = wF[i];
-            }
-            if (SUB == 2) {
-              uint32_t wO[KW], wQ[KW];
-              Pack<ST, K>(ov, wO);
-              Pack<ST, K>(qv, wQ);
-              StoreWords<ST, K>(row + 3 * stride + c0, wO);
-              StoreWords<ST, K>(row + 4 * stride + c0, wQ);
-              if (single) {
-#pragma unroll
-                for (int i = 0; i < KW; ++i) cO[i] = wO[i];
+            const int32_t hs = type == 0 ? max(h, 0) : h;
+            if constexpr (CPW == 2) {
+              if (k & 1) {
+                hw[k >> 1] = (static_cast<uint32_t>(hlo) & 0xffffu) | (static_cast<uint32_t>(hs) << 16);
+                if (SUB >= 1) ew[k >> 1] = (static_cast<uint32_t>(elo) & 0xffffu) | (static_cast<uint32_t>(ek) << 16);
+                if (SUB == 2) qw[k >> 1] = (static_cast<uint32_t>(qlo) & 0xffffu) | (static_cast<uint32_t>(qk) << 16);
+              } else {
+                hlo = hs;
+                elo = ek;
+                qlo = qk;
               }
+            } else {
+              hw[k] = static_cast<uint32_t>(hs);
+              if (SUB >= 1) ew[k] = static_cast<uint32_t>(ek);
+              if (SUB == 2) qw[k] = static_cast<uint32_t>(qk);
             }
           }
-          uint32_t wH[KW];
-          Pack<ST, K>(hs, wH);
-          StoreWords<ST, K>(row + c0, wH);
-          if (single) {
-#pragma unroll
-            for (int i = 0; i < KW; ++i) cH[i] = wH[i];
+          if (active) {
+            BufStore<KW>(rrow, off, hw);
+            if (SUB >= 1) BufStore<KW>(rrow, off + 2 * mbytes, ew);
+            if (SUB == 2) BufStore<KW>(rrow, off + 4 * mbytes, qw);
           }
+        }
+        if (single) {
+#pragma unroll
+          for (int i = 0; i < KW; ++i) cH[i] = hw[i];
+        }
 
-          // end-cell candidates (:903-928): SW all rows, OV sink rows (row
-          // maxima floored at 0, :252-262), NW sink rows at column n - 1
+        // end-cell candidates (:903-928): SW all rows, OV sink rows (row
+        // maxima floored at 0, :252-262), NW sink rows at column n - 1
+        if (active) {
           if (type == 0 || (type == 2 && sink)) {
-            int32_t lm = 0;
+            // whole words up to the reference width W (a multiple of 8)
+            const int32_t wv = (static_cast<int32_t>(W) - static_cast<int32_t>(c0)) / CPW;
+            uint32_t mw = Splat<ST>(0);
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-              if (c0 + k < W) lm = max(lm, hs[k]);
+            for (int i = 0; i < KW; ++i) {
+              if (i < wv) mw = WMax<ST>(mw, hw[i]);
+            }
+            int32_t lm;
+            if constexpr (CPW == 2) {
+              lm = max(static_cast<int32_t>(static_cast<int16_t>(mw)), static_cast<int32_t>(mw) >> 16);
+            } else {
+              lm = static_cast<int32_t>(mw);
             }
             if (best < lm) {
               best = lm;
@@ -817,10 +897,10 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
           } else if (type == 1 && sink) {
             const uint32_t cn = n - 1;
             if (cn >= c0 && cn < c0 + K) {
-              int32_t v = hs[0];
+              int32_t v = Col<ST, KW>(hw, 0);
 #pragma unroll
               for (int k = 1; k < K; ++k) {
-                if (c0 + k == cn) v = hs[k];
+                if (c0 + k == cn) v = Col<ST, KW>(hw, k);
               }
               if (best < v) {
                 best = v;
@@ -871,11 +951,12 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
 
     // max_j (:937-953): NW the last column; SW/OV the first column of row
     // max_i (padding included) holding the best value, -1 if none
+    const uint64_t rstride = static_cast<uint64_t>(stride) * NM;
     int32_t max_j = static_cast<int32_t>(n) - 1;
     if (found && type != 1) {
       const ST* __restrict__ hrow = mat + static_cast<uint64_t>(max_i) * rstride;
       int32_t first = 0x7fffffff;
-      for (uint32_t col = tid; col < W; col += blockDim.x) {
+      for (uint32_t col = tid; col < W; col += nthreads) {
         if (static_cast<int32_t>(hrow[col]) == max_score) {
           first = static_cast<int32_t>(col);
           break;

@@ -11,10 +11,11 @@
 // Error model (per source base, probability p): substitution to one of the
 // three other bases, insertion of a uniform random base before the source
 // base, or deletion, each with probability 1/3.  Read sources are either
-// the whole template or a substring of the template read as a circular
-// sequence (so reads longer than the template wrap around it).
+// the whole template or a substring of it at a uniform offset, of the drawn
+// length clipped to the template length.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -135,9 +136,11 @@ inline Window MakeWindow(std::uint64_t seed, const WindowSpec& sp) {
       } else {
         len = rng.Int(sp.read_len_min, sp.read_len_max);
       }
-      std::uint32_t off = rng.Int(0, tlen - 1);
-      src.resize(len);
-      for (std::uint32_t k = 0; k < len; ++k) src[k] = w.templ[(off + k) % tlen];
+      // a substring of the template; reads longer than the template are
+      // the whole template
+      len = std::min(len, tlen);
+      const std::uint32_t off = rng.Int(0, tlen - len);
+      src = w.templ.substr(off, len);
     }
     std::string read = Mutate(&rng, src, w.err);
     if (read.empty()) read.push_back(rng.Base());
