@@ -54,6 +54,18 @@ int spoa_graph_msa(void* graph, int include_consensus, char* out, uint64_t cap, 
 /* rank_to_node() ids (graph.hpp:110-112) */
 int spoa_graph_ranks(void* graph, uint32_t* out, uint32_t cap);
 
+/* The flat partial-order graph of the batched form: the same operations
+ * and results as spoa::Graph (AddAlignment, GenerateConsensus,
+ * GenerateMultipleSequenceAlignment, rank_to_node). */
+void* spoa_poa_graph_new(void);
+void spoa_poa_graph_free(void* graph);
+uint32_t spoa_poa_graph_num_nodes(void* graph);
+int spoa_poa_graph_add_alignment(void* graph, const int32_t* pairs, uint32_t n_pairs, const char* seq, uint32_t len,
+                                 const uint32_t* weights, uint32_t n_weights);
+int spoa_poa_graph_consensus(void* graph, char* out, uint32_t cap);
+int spoa_poa_graph_msa(void* graph, int include_consensus, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols);
+int spoa_poa_graph_ranks(void* graph, uint32_t* out, uint32_t cap);
+
 /* ----------------------------------------------------------- windows -- */
 enum { SPOA_BATCH_BATCHED = 0, SPOA_BATCH_PER_CALL = 1 };
 

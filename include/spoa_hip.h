@@ -152,6 +152,9 @@ int spoa_hip_set_config(spoa_hip_engine* engine, int32_t width_bits, int32_t bat
  * not fit is split into launches that do; a single job that does not fit
  * fails with SPOA_HIP_NO_MEMORY ("insufficient memory!"). */
 int spoa_hip_set_memory_budget(spoa_hip_engine* engine, uint64_t bytes);
+/* The current budget (by default 80% of the device memory free when the
+ * engine was created). */
+uint64_t spoa_hip_memory_budget(spoa_hip_engine* engine);
 
 /* DP-matrix bytes one job needs (the admission footprint used by the
  * serving loop), or 0 when the job runs no DP. */

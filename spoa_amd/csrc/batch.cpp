@@ -7,6 +7,7 @@
 #include <chrono>
 #include <stdexcept>
 
+#include "poa_graph.hpp"
 #include "spoa_amd_host.hpp"
 
 namespace spoa_amd {
@@ -118,6 +119,11 @@ void ThreadPool::For(std::size_t n, const std::function<void(std::size_t)>& fn) 
 
 BatchContext::BatchContext(const BatchOptions& opt) : opt_(opt), pool_(opt.threads) {
   for (auto& e : engines_) e = spoa::AlignmentEngine::Create(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c);
+  // the halves share the device memory: each engine gets half the budget
+  for (auto& e : engines_) {
+    auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(e.get());
+    if (hip) spoa_hip_set_memory_budget(hip->device(), spoa_hip_memory_budget(hip->device()) / 2);
+  }
 }
 
 std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats) {
@@ -148,10 +154,15 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
   const BatchOptions& opt = ctx.options();
   ThreadPool& pool = ctx.pool();
   const std::uint32_t W = ws.n_windows;
-  std::vector<spoa::Graph> graphs(W);
+  std::vector<PoaGraph> graphs(W);
   std::vector<WindowResult> out(W);
   std::vector<std::string> rc_seq(W);
   const int per_read = opt.strand ? 2 : 1;
+  std::uint64_t max_len = 0;
+  for (std::uint64_t k = 0; k + 1 < ws.win_read_off[W] + 1ull; ++k) {
+    max_len = std::max<std::uint64_t>(max_len, ws.read_off[k + 1] - ws.read_off[k]);
+  }
+  const std::vector<std::uint32_t> ones(max_len, 1u);
 
   // halves balanced by total read bases, largest windows first
   Half half[2];
@@ -206,10 +217,10 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
       const std::uint32_t w = hf.live[x];
       std::uint32_t len;
       const char* s = read_of(w, r, &len);
-      if (graphs[w].nodes().empty() || len == 0) continue;
+      if (graphs[w].num_nodes() == 0 || len == 0) continue;
       for (int d = 0; d < per_read; ++d) {
         spoa_hip_job j{};
-        j.graph = spoa::HipAlignmentEngine::View(graphs[w]);
+        j.graph = graphs[w].View();
         j.seq = d == 0 ? s : rc_seq[w].data();
         j.len = len;
         const std::uint32_t cap = j.graph.num_nodes + len;
@@ -266,23 +277,25 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
         const std::uint32_t w = hf.live[x];
         std::uint32_t len;
         const char* s = read_of(w, r, &len);
-        auto alignment_of = [&](int d, std::int32_t* score) {
-          spoa::Alignment a;
+        // the job's pairs (path order), or none where no DP ran
+        auto alignment_of = [&](int d, std::int32_t* score, std::uint32_t* n) -> const std::int32_t* {
+          *n = 0;
           const std::int32_t ji = hf.job_of[x * per_read + d];
-          if (ji < 0) return a;
+          if (ji < 0) return nullptr;
           const spoa_hip_job& j = hf.jobs[ji];
-          a.resize(j.n_pairs);
-          for (std::uint32_t p = 0; p < j.n_pairs; ++p) a[p] = {j.pairs[2 * p], j.pairs[2 * p + 1]};
           if (j.has_score) *score = j.score;
-          return a;
+          *n = j.n_pairs;
+          return j.pairs;
         };
         std::int32_t score = 0, score_rev = 0;
-        spoa::Alignment a = alignment_of(0, &score);
+        std::uint32_t na = 0, nb = 0;
+        const std::int32_t* a = alignment_of(0, &score, &na);
         bool reversed = false;
         if (opt.strand) {
-          spoa::Alignment b = alignment_of(1, &score_rev);
+          const std::int32_t* b = alignment_of(1, &score_rev, &nb);
           if (!(score >= score_rev)) {
-            a.swap(b);
+            a = b;
+            na = nb;
             reversed = true;
           }
           out[w].reversed.push_back(reversed ? 1 : 0);
@@ -292,9 +305,9 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
           const char* q = ws.quals + (s - ws.seqs);
           std::vector<std::uint32_t> wt = Weights(q, len);
           if (reversed) std::reverse(wt.begin(), wt.end());
-          graphs[w].AddAlignment(a, seq, len, wt);
+          graphs[w].AddAlignment(a, na, seq, len, wt.data());
         } else {
-          graphs[w].AddAlignment(a, seq, len);
+          graphs[w].AddAlignment(a, na, seq, len, ones.data());
         }
       } catch (const std::exception& ex) {
         std::lock_guard<std::mutex> lk(err_mu);

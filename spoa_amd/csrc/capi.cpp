@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "poa_graph.hpp"
 #include "spoa/spoa.hpp"
 #include "spoa_amd_host.hpp"
 #include "synth.hpp"
@@ -170,6 +171,56 @@ int spoa_graph_ranks(void* graph, uint32_t* out, uint32_t cap) {
   const auto& r = Gr(graph)->rank_to_node();
   if (r.size() > cap) return -2;
   for (std::size_t k = 0; k < r.size(); ++k) out[k] = r[k]->id;
+  return static_cast<int>(r.size());
+}
+
+// ----------------------------------------------------- flat POA graph --
+
+void* spoa_poa_graph_new(void) { return new spoa_amd::PoaGraph(); }
+void spoa_poa_graph_free(void* g) { delete static_cast<spoa_amd::PoaGraph*>(g); }
+uint32_t spoa_poa_graph_num_nodes(void* g) { return static_cast<spoa_amd::PoaGraph*>(g)->num_nodes(); }
+
+int spoa_poa_graph_add_alignment(void* g, const int32_t* pairs, uint32_t n_pairs, const char* seq, uint32_t len,
+                                 const uint32_t* weights, uint32_t n_weights) {
+  try {
+    if (weights && n_weights != len) {
+      throw std::invalid_argument(
+          "[spoa::Graph::AddAlignment] error: "
+          "sequence and weights are of unequal size!");
+    }
+    std::vector<uint32_t> ones;
+    if (!weights) {
+      ones.assign(len, 1u);
+      weights = ones.data();
+    }
+    static_cast<spoa_amd::PoaGraph*>(g)->AddAlignment(pairs, n_pairs, seq, len, weights);
+    return 0;
+  } catch (const std::exception& ex) {
+    return Error(ex);
+  }
+}
+
+int spoa_poa_graph_consensus(void* g, char* out, uint32_t cap) {
+  const std::string s = static_cast<spoa_amd::PoaGraph*>(g)->GenerateConsensus();
+  if (s.size() > cap) return -2;
+  std::memcpy(out, s.data(), s.size());
+  return static_cast<int>(s.size());
+}
+
+int spoa_poa_graph_msa(void* g, int include_consensus, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols) {
+  auto m = static_cast<spoa_amd::PoaGraph*>(g)->GenerateMultipleSequenceAlignment(include_consensus != 0);
+  *rows = static_cast<uint32_t>(m.size());
+  *cols = m.empty() ? 0 : static_cast<uint32_t>(m[0].size());
+  const uint64_t need = static_cast<uint64_t>(*rows) * *cols;
+  if (need > cap) return -2;
+  for (std::size_t r = 0; r < m.size(); ++r) std::memcpy(out + r * (*cols), m[r].data(), *cols);
+  return 0;
+}
+
+int spoa_poa_graph_ranks(void* g, uint32_t* out, uint32_t cap) {
+  const auto& r = static_cast<spoa_amd::PoaGraph*>(g)->rank_to_node();
+  if (r.size() > cap) return -2;
+  std::memcpy(out, r.data(), r.size() * sizeof(uint32_t));
   return static_cast<int>(r.size());
 }
 

@@ -307,8 +307,10 @@ Cfg DefaultCfg(const spoa_hip_engine* E, uint32_t W, int width_bits, bool batch)
   Cfg c;
   const bool convex16 = E->prm.subtype == 2 && width_bits == 16;
   const int kmin = convex16 ? 16 : 8;
+  // int32 rows hold twice the registers of int16 rows: at most 16 columns a lane
+  const int kmax = width_bits == 32 ? 16 : 32;
   if (batch) {
-    for (int k = kmin; k <= 32; k *= 2) {
+    for (int k = kmin; k <= kmax; k *= 2) {
       if (W <= 64u * static_cast<uint32_t>(k)) {
         c.k = k;
         c.waves = 1;
@@ -958,6 +960,8 @@ int spoa_hip_set_memory_budget(spoa_hip_engine* E, uint64_t bytes) {
   E->budget = bytes;
   return SPOA_HIP_OK;
 }
+
+uint64_t spoa_hip_memory_budget(spoa_hip_engine* E) { return E ? E->budget : 0; }
 
 uint64_t spoa_hip_job_footprint(spoa_hip_engine* E, uint32_t V, uint32_t len, int32_t batch) {
   if (!E || V == 0 || len == 0) return 0;

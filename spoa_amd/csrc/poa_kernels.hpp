@@ -458,8 +458,9 @@ __device__ __forceinline__ void ProfileWords(const uint32_t (&scp)[K / 4], int32
 // per lane.  MAXT: launch bound (threads per workgroup).
 //
 // Dynamic LDS: the job's query profile, 4 row bases x K/2 words per lane
-// (8 * K * blockDim.x bytes), used when a row is one tile and the graph
-// has at most 4 distinct bases (the profile is otherwise formed per row).
+// (8 * K * blockDim.x bytes, each lane's words contiguous), used when a row
+// is one tile and the graph has at most 4 distinct bases (the profile is
+// otherwise formed per row).
 template <int SUB, typename ST, int K, int MAXT>
 __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
   constexpr int L = sizeof(ST) == 2 ? 16 : 8;
@@ -474,6 +475,7 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
   __shared__ int32_t s_tot[2][kMaxWaves][2];
   __shared__ int32_t s_red[kMaxWaves][3];
   __shared__ uint32_t s_job;
+  __shared__ int32_t s_mpow[7][4];  // convex scan matrices (kept out of scalar registers)
   extern __shared__ uint32_t s_prof[];
 
   const KParams& prm = a.prm;
@@ -486,6 +488,7 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
   const int32_t m = prm.m, mis = prm.mis, g = prm.g, e = prm.e, q = prm.q, cc = prm.c;
   const int32_t pad = prm.pad;
   const uint32_t G2 = Splat<ST>(g), E2 = Splat<ST>(e), Q2 = Splat<ST>(q), C2 = Splat<ST>(cc);
+  if (SUB == 2 && tid < 28) s_mpow[tid / 4][tid % 4] = prm.mpow[tid / 4][tid % 4];
 
   for (;;) {
     if (tid == 0) s_job = atomicAdd(a.queue, 1u);
@@ -494,8 +497,22 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
     __syncthreads();
     if (jx >= a.n_jobs) break;
 
-    const DevJob J = a.jobs[jx];
-    const uint32_t V = Uniform(J.V), n = Uniform(J.n), W = Uniform(J.W), stride = Uniform(J.stride);
+    // the job record, every field uniform (scalar registers)
+    const DevJob Jv = a.jobs[jx];
+    DevJob J;
+    J.mat_addr = (static_cast<uint64_t>(Uniform(static_cast<uint32_t>(Jv.mat_addr >> 32))) << 32) |
+                 Uniform(static_cast<uint32_t>(Jv.mat_addr));
+    J.graph_off = Uniform(Jv.graph_off);
+    J.seq_off = Uniform(Jv.seq_off);
+    J.V = Uniform(Jv.V);
+    J.n = Uniform(Jv.n);
+    J.W = Uniform(Jv.W);
+    J.stride = Uniform(Jv.stride);
+    J.pair_off = Uniform(Jv.pair_off);
+    J.cap = Uniform(Jv.cap);
+    J.nslots = Uniform(Jv.nslots);
+    J.slot_chars = Uniform(Jv.slot_chars);
+    const uint32_t V = J.V, n = J.n, W = J.W, stride = J.stride;
     const GraphView gv = MakeView(a.graphs + J.graph_off, V);
     const uint8_t* __restrict__ seq = a.seqs + J.seq_off;
     ST* __restrict__ mat = reinterpret_cast<ST*>(J.mat_addr);
@@ -506,7 +523,7 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
     const uint32_t ntiles = (stride + tile_w - 1) / tile_w;
     const bool single = ntiles == 1;  // a row is one tile: the previous row stays in registers
     const bool sync = nwaves > 1 || ntiles > 1;
-    const uint32_t nslots = single ? Uniform(J.nslots) : 0u;
+    const uint32_t nslots = single ? J.nslots : 0u;
     auto row_ptr = [&](uint32_t r) -> ST* { return mat + static_cast<uint64_t>(r) * stride * NM; };
 
     // row 0 (:556-694): E = g + c*e, Q = q + c*c, F = O = -inf, H per type
@@ -539,7 +556,7 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
       uint32_t pw[PW];
       ProfileWords<K>(scp, nvalid, (J.slot_chars >> (8 * sl)) & 0xffu, m, mis, pad, pw);
 #pragma unroll
-      for (int i = 0; i < PW; ++i) s_prof[(sl * PW + i) * nthreads + tid] = pw[i];
+      for (int i = 0; i < PW; ++i) s_prof[(sl * nthreads + tid) * PW + i] = pw[i];
     }
 
     // the previous row (H, F, O words of this lane's columns), from row 0
@@ -565,14 +582,23 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
     int32_t best_row = -1;
     int par = 0;
     int32_t fch_prev = gv.fcH[0];
+    // row records two ahead, and the first predecessor row of the next row
+    // when it is not the current one: loaded before the current row's
+    // stores, so waiting for it never waits for those stores
     uint4 rec = gv.rows[0];
+    uint4 rec2 = V > 1 ? gv.rows[1] : make_uint4(0u, 0u, 0u, 0u);
+    uint32_t pfH[KW], pfF[KW], pfO[KW];
+    bool pf_valid = false;
+    uint32_t pf_row = 0;
 
     for (uint32_t r = 1; r <= V; ++r) {
       // row record: base | sink << 8 | profile slot << 9 | predecessor count
       // << 12, first predecessor row, offset of the predecessor list,
-      // first-column score (the next record is prefetched)
+      // first-column score
       const uint4 cur = rec;
-      if (r < V) rec = gv.rows[r];
+      const uint4 nxt = rec2;  // row r + 1 (when r < V)
+      rec = rec2;
+      if (r + 1 < V) rec2 = gv.rows[r + 1];
       const uint32_t rch = cur.x & 0xffu;
       const bool sink = ((cur.x >> 8) & 1u) != 0;
       const uint32_t slot = (cur.x >> 9) & 7u;
@@ -594,8 +620,15 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
         if (!single) LoadChars<K>(seq, c0, n, active, scp, nvalid);
         uint32_t pw[PW];
         if (slot < nslots) {
+          const uint4* src = reinterpret_cast<const uint4*>(s_prof + (slot * nthreads + tid) * PW);
 #pragma unroll
-          for (int i = 0; i < PW; ++i) pw[i] = s_prof[(slot * PW + i) * nthreads + tid];
+          for (int i = 0; i < PW / 4; ++i) {
+            const uint4 v = src[i];
+            pw[4 * i + 0] = v.x;
+            pw[4 * i + 1] = v.y;
+            pw[4 * i + 2] = v.z;
+            pw[4 * i + 3] = v.w;
+          }
         } else {
           ProfileWords<K>(scp, nvalid, rch, m, mis, pad, pw);
         }
@@ -605,11 +638,15 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
         for (uint32_t pi = 0; pi < npred; ++pi) {
           const uint32_t p = pi == 0 ? p0 : Uniform(gv.preds[pb + pi]);
           const bool cached = single && p + 1 == r;
+          const bool fetched = !cached && pi == 0 && pf_valid && pf_row == p;
           const __amdgpu_buffer_rsrc_t rp = RowRsrc(row_ptr(p), rbytes);
           uint32_t hv[KW];
           if (cached) {
 #pragma unroll
             for (int i = 0; i < KW; ++i) hv[i] = cH[i];
+          } else if (fetched) {
+#pragma unroll
+            for (int i = 0; i < KW; ++i) hv[i] = pfH[i];
           } else {
             BufLoad<KW>(rp, active ? off : rbytes, hv);
           }
@@ -647,6 +684,9 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
             if (cached) {
 #pragma unroll
               for (int i = 0; i < KW; ++i) fp[i] = cF[i];
+            } else if (fetched) {
+#pragma unroll
+              for (int i = 0; i < KW; ++i) fp[i] = pfF[i];
             } else {
               BufLoad<KW>(rp, active ? off + mbytes : rbytes, fp);
             }
@@ -657,6 +697,9 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
               if (cached) {
 #pragma unroll
                 for (int i = 0; i < KW; ++i) op[i] = cO[i];
+              } else if (fetched) {
+#pragma unroll
+                for (int i = 0; i < KW; ++i) op[i] = pfO[i];
               } else {
                 BufLoad<KW>(rp, active ? off + 3 * mbytes : rbytes, op);
               }
@@ -685,6 +728,16 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
         for (int i = 0; i < KW; ++i) {
           h1[i] = WMax<ST>(h1[i], fv[i]);
           if (SUB == 2) h1[i] = WMax<ST>(h1[i], ov[i]);
+        }
+        // prefetch the next row's first predecessor unless it is this row
+        pf_valid = false;
+        if (single && r < V && nxt.y != r) {
+          const __amdgpu_buffer_rsrc_t rq = RowRsrc(row_ptr(nxt.y), rbytes);
+          BufLoad<KW>(rq, active ? off : rbytes, pfH);
+          if (SUB >= 1) BufLoad<KW>(rq, active ? off + mbytes : rbytes, pfF);
+          if (SUB == 2) BufLoad<KW>(rq, active ? off + 3 * mbytes : rbytes, pfO);
+          pf_valid = true;
+          pf_row = nxt.y;
         }
         if (SUB >= 1 && active) {
           BufStore<KW>(rrow, off + mbytes, fv);
@@ -748,7 +801,7 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
             int32_t z0 = __shfl_up(s0, sh, 64);
             int32_t z1 = __shfl_up(s1, sh, 64);
             if (lane >= sh) {
-              MpApply(prm.mpow[st], z0, z1);
+              MpApply(s_mpow[st], z0, z1);
               s0 = max(s0, z0);
               s1 = max(s1, z1);
             }
@@ -773,7 +826,7 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
             if (SUB != 2) {
               f0 = max(t0, f0 + 64 * prm.decay);
             } else {
-              MpApply(prm.mpow[6], f0, f1);
+              MpApply(s_mpow[6], f0, f1);
               f0 = max(f0, t0);
               f1 = max(f1, t1);
             }
@@ -787,7 +840,7 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
           if (SUB != 2) {
             f0 = max(t0, f0 + 64 * prm.decay);
           } else {
-            MpApply(prm.mpow[6], f0, f1);
+            MpApply(s_mpow[6], f0, f1);
             f0 = max(f0, t0);
             f1 = max(f1, t1);
           }
@@ -805,16 +858,21 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
           int32_t u0 = w0, u1 = w1;
 #pragma unroll
           for (int st = 0; st < 6; ++st) {
-            if ((lane >> st) & 1) MpApply(prm.mpow[st], u0, u1);
+            if ((lane >> st) & 1) MpApply(s_mpow[st], u0, u1);
           }
           x0 = lane == 0 ? w0 : max(e0, u0);
           x1 = lane == 0 ? w1 : max(e1, u1);
         }
 
         // ---- 3. exact pass: E, Q, H column by column, packed as produced
+        // (the H1 words are re-read here rather than keeping their unpacked
+        // columns alive across the scan)
+#pragma unroll
+        for (int i = 0; i < KW; ++i) asm volatile("" : "+v"(h1[i]));
         uint32_t hw[KW];
         {
-          uint32_t ew[KW], qw[KW];
+          // E and Q leave in 16-byte chunks as soon as a chunk is complete
+          uint32_t ew[4], qw[4];
           int32_t pe = x0, pq = x1;  // E, Q at the previous column (before column c0: the carry)
           int32_t hlo = 0, elo = 0, qlo = 0;
 #pragma unroll
@@ -846,27 +904,31 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
               pq = qk;
             }
             const int32_t hs = type == 0 ? max(h, 0) : h;
+            const int wi = k / CPW;  // word of column k
+            bool word_done = true;
             if constexpr (CPW == 2) {
               if (k & 1) {
-                hw[k >> 1] = (static_cast<uint32_t>(hlo) & 0xffffu) | (static_cast<uint32_t>(hs) << 16);
-                if (SUB >= 1) ew[k >> 1] = (static_cast<uint32_t>(elo) & 0xffffu) | (static_cast<uint32_t>(ek) << 16);
-                if (SUB == 2) qw[k >> 1] = (static_cast<uint32_t>(qlo) & 0xffffu) | (static_cast<uint32_t>(qk) << 16);
+                hw[wi] = (static_cast<uint32_t>(hlo) & 0xffffu) | (static_cast<uint32_t>(hs) << 16);
+                if (SUB >= 1) ew[wi % 4] = (static_cast<uint32_t>(elo) & 0xffffu) | (static_cast<uint32_t>(ek) << 16);
+                if (SUB == 2) qw[wi % 4] = (static_cast<uint32_t>(qlo) & 0xffffu) | (static_cast<uint32_t>(qk) << 16);
               } else {
                 hlo = hs;
                 elo = ek;
                 qlo = qk;
+                word_done = false;
               }
             } else {
-              hw[k] = static_cast<uint32_t>(hs);
-              if (SUB >= 1) ew[k] = static_cast<uint32_t>(ek);
-              if (SUB == 2) qw[k] = static_cast<uint32_t>(qk);
+              hw[wi] = static_cast<uint32_t>(hs);
+              if (SUB >= 1) ew[wi % 4] = static_cast<uint32_t>(ek);
+              if (SUB == 2) qw[wi % 4] = static_cast<uint32_t>(qk);
+            }
+            if (SUB >= 1 && word_done && wi % 4 == 3 && active) {
+              const uint32_t at = off + 16u * (wi / 4);
+              BufStore<4>(rrow, at + 2 * mbytes, ew);
+              if (SUB == 2) BufStore<4>(rrow, at + 4 * mbytes, qw);
             }
           }
-          if (active) {
-            BufStore<KW>(rrow, off, hw);
-            if (SUB >= 1) BufStore<KW>(rrow, off + 2 * mbytes, ew);
-            if (SUB == 2) BufStore<KW>(rrow, off + 4 * mbytes, qw);
-          }
+          if (active) BufStore<KW>(rrow, off, hw);
         }
         if (single) {
 #pragma unroll
