@@ -15,6 +15,13 @@ MSA.  One step runs the whole batch of windows through the batched form
 Prints one JSON line.  Inputs are resident on the host when a step starts
 and every step ends with all device work complete (the window driver
 synchronises its stream), so the timed region brackets whole-job work.
+
+Roofline: the alignment launches run one after another on one compute
+stream (the two halves of the windows take turns), each timed by HIP events
+recorded around it on that stream; achieved = algorithmic DP bytes per
+launch / average launch duration.  The HBM traffic per launch comes from
+the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same
+workload (profiles/traffic.json), when one matches.
 """
 import argparse
 import concurrent.futures
@@ -32,15 +39,37 @@ import spoa_amd  # noqa: E402
 PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline(windows, params, strand, budget_s, threads):
-    """Times the oracle on the host cores over as many windows as fit about
-    budget_s seconds (at least one per thread).  Returns a cpu_baseline
-    record, or None where the oracle library is absent."""
-    so = os.path.join(ROOT, "oracle", "liboracle.so")
-    if not os.path.exists(so):
+def measured_traffic(workload, n_windows):
+    """HBM bytes per alignment launch measured by rocprofv3 for this
+    workload (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(workload)
+    except (OSError, ValueError):
         return None
-    lib = ctypes.CDLL(so)
-    fn = lib.oracle_run_window
+    if not rec or rec.get("windows_per_step") != n_windows:
+        return None
+    return rec.get("traffic_bytes_per_launch")
+
+
+def cpu_baseline(windows, params, strand, budget_s, threads):
+    """Times a CPU engine on the host cores over as many windows as fit
+    about budget_s seconds (at least one per thread): the reference itself
+    (its AVX2 engine, compiled from its sources into oracle/_ref) where that
+    library was built, else the oracle (the repository's scalar C++
+    restatement).  Returns a cpu_baseline record, or None where neither
+    library is present."""
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libspoa_ref.so")
+    port_so = os.path.join(ROOT, "oracle", "liboracle.so")
+    if os.path.exists(ref_so):
+        kind, what = "reference", "the reference (spoa v4.0.8 SimdAlignmentEngine, AVX2 build)"
+        fn = ctypes.CDLL(ref_so).ref_run_window
+    elif os.path.exists(port_so):
+        kind, what = "port", "the oracle (scalar C++ restatement of the reference)"
+        fn = ctypes.CDLL(port_so).oracle_run_window
+    else:
+        return None
     fn.restype = ctypes.c_int
     fn.argtypes = [ctypes.c_int] * 7 + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int,
                                         ctypes.c_char_p, ctypes.c_uint32]
@@ -81,9 +110,9 @@ def cpu_baseline(windows, params, strand, budget_s, threads):
                     futs[ex.submit(one, nxt)] = nxt
                     nxt += 1
     el = time.perf_counter() - t0
-    return {"value": done / el, "unit": "windows/s", "cores": threads, "kind": "port",
-            "sample": "%d windows of the benchmark workload through the oracle (C++ restatement of the "
-                      "reference, scalar) on %d host threads, %.1f s" % (done, threads, el),
+    return {"value": done / el, "unit": "windows/s", "cores": threads, "kind": kind,
+            "sample": "%d windows of the benchmark workload (Align + AddAlignment per read, consensus, MSA) "
+                      "through %s, one window per thread, %d host threads, %.1f s" % (done, what, threads, el),
             "consensus": results}
 
 
@@ -117,7 +146,7 @@ def main():
         del src, dst
 
     cfg = spoa_amd.CONFIGS[args.config]
-    n_windows = args.windows or {1: 1024, 2: 512, 3: 4096, 4: 64}[args.config]
+    n_windows = args.windows or {1: 1024, 2: 2048, 3: 4096, 4: 64}[args.config]
     threads = args.threads or min(16, os.cpu_count() or 1)
     params = cfg["params"]
     strand = cfg.get("strand", False)
@@ -182,7 +211,7 @@ def main():
         "gcups_dp": cells / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else 0.0,
         "gcups_end_to_end": cells / el / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(cfg["name"], n_windows),
                      "kernel": "PoaAlignKernel (batched launches)", "launches": launches,
                      "avg_launch_ms": kernel_ms / max(1, launches),
                      "bytes_per_launch": dp_bytes / max(1, launches)},

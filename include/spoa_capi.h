@@ -75,6 +75,7 @@ typedef struct spoa_batch_opts {
   int32_t msa;     /* also build every window's MSA */
   int32_t threads; /* host threads, 0 = all */
   int32_t device;
+  int32_t overlap; /* batched form: the two halves' launches may run on the device at once */
 } spoa_batch_opts;
 
 typedef struct spoa_batch_stats {
@@ -98,10 +99,18 @@ void spoa_batch_free(void* result);
 void* spoa_batch_context_new(const spoa_batch_opts* opts);
 void spoa_batch_context_free(void* ctx);
 spoa_hip_engine* spoa_batch_context_device(void* ctx);
+/* Launch configuration of the context's batched launches for rows of
+ * reference width up to max_width at one score width (spoa_hip_set_config
+ * on both engines); cols_per_lane 0 clears that width's overrides. */
+int spoa_batch_context_set_config(void* ctx, int32_t width_bits, uint32_t max_width, int32_t cols_per_lane,
+                                  int32_t waves);
 void* spoa_batch_context_run(void* ctx, uint32_t n_windows, const uint32_t* win_read_off, const uint64_t* read_off,
                              const char* seqs, const char* quals);
 int spoa_batch_get_stats(void* result, spoa_batch_stats* out);
 int spoa_batch_consensus(void* result, uint32_t window, char* out, uint32_t cap);
+/* every window's consensus back to back; offs[n_windows + 1] gets the
+ * offsets (filled even when cap is too small); returns the total length */
+int64_t spoa_batch_consensus_all(void* result, char* out, uint64_t cap, uint64_t* offs);
 int spoa_batch_msa(void* result, uint32_t window, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols);
 int spoa_batch_reversed(void* result, uint32_t window, uint8_t* out, uint32_t cap);
 

@@ -120,7 +120,7 @@ int spoa_hip_align_batch_wait(spoa_hip_engine* engine);
  * S * n * (W * V + Rd * R): every score matrix written once, the
  * predecessor rows read once per in-edge). */
 typedef struct spoa_hip_stats {
-  double kernel_ms;  /* device time of the alignment launches (HIP events) */
+  double kernel_ms;  /* sum of the alignment launches' device times (HIP events around each launch) */
   double h2d_ms;     /* device time of the input copy */
   double d2h_ms;     /* device time of the output copy */
   double host_ms;    /* host time packing the jobs */
@@ -142,12 +142,19 @@ int spoa_hip_config(spoa_hip_engine* engine, uint32_t len, int32_t width_bits, i
 
 /* Override the launch configuration for rows whose reference width
  * (ceil(len / L) * L) is at most max_width, at one score width and mode;
- * used by the autotuner.  cols_per_lane in {8, 16} (convex int16 needs 16),
- * waves in 1..8.  cols_per_lane == 0 clears all overrides of that width
+ * used by the autotuner.  cols_per_lane in {8, 16, 32} (convex int16 needs
+ * at least 16; 32 with one wave only), waves in 1..8.  cols_per_lane == 0 clears all overrides of that width
  * and mode. */
 int spoa_hip_set_config(spoa_hip_engine* engine, int32_t width_bits, int32_t batch, uint32_t max_width,
                         int32_t cols_per_lane, int32_t waves);
 
+/* No counterpart in the reference.  Makes engine launch its kernels on
+ * owner's compute stream (owner NULL: back on its own), so the kernels of
+ * several engines run one after another while their copies and host work
+ * overlap; every launch then has the device to itself, which is what the
+ * per-launch times in spoa_hip_stats measure.  Both engines must be on one
+ * device, and owner must outlive engine. */
+int spoa_hip_share_compute(spoa_hip_engine* engine, spoa_hip_engine* owner);
 /* Device memory budget for DP matrices (bytes).  A batch whose matrices do
  * not fit is split into launches that do; a single job that does not fit
  * fails with SPOA_HIP_NO_MEMORY ("insufficient memory!"). */

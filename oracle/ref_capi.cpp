@@ -9,6 +9,7 @@
 // GenerateConsensus / GenerateMultipleSequenceAlignment / rank_to_node
 // (include/spoa/graph.hpp:106-171).
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -121,6 +122,56 @@ int ref_graph_ranks(void* graph, std::uint32_t* out, std::uint32_t cap) {
   if (r.size() > cap) return -2;
   for (std::size_t k = 0; k < r.size(); ++k) out[k] = r[k]->id;
   return static_cast<int>(r.size());
+}
+
+// One window through the reference: every read aligned and added in order
+// (strand mode: the read or its reverse complement, whichever aligns with
+// the higher score, forward on ties, as src/main.cpp:285-302 does), then
+// the consensus and the MSA.  Writes the consensus; returns its length,
+// -2 if cap is too small, -1 on error.  The CPU-baseline timing entry.
+int ref_run_window(int type, int m, int n, int g, int e, int q, int c, std::uint32_t n_reads,
+                   const std::uint64_t* read_off, const char* seqs, int strand, char* out, std::uint32_t cap) {
+  try {
+    auto eng = spoa::AlignmentEngine::Create(static_cast<spoa::AlignmentType>(type), m, n, g, e, q, c);
+    spoa::Graph gr;
+    for (std::uint32_t k = 0; k < n_reads; ++k) {
+      const char* s = seqs + read_off[k];
+      const std::uint32_t len = static_cast<std::uint32_t>(read_off[k + 1] - read_off[k]);
+      std::int32_t sf = 0;
+      auto a = eng->Align(s, len, gr, &sf);
+      if (!strand) {
+        gr.AddAlignment(a, s, len);
+        continue;
+      }
+      std::string rev(s, len);
+      std::reverse(rev.begin(), rev.end());
+      for (auto& b : rev) {
+        switch (b) {
+          case 'A': b = 'T'; break;
+          case 'C': b = 'G'; break;
+          case 'G': b = 'C'; break;
+          case 'T': b = 'A'; break;
+          default: break;
+        }
+      }
+      std::int32_t sr = 0;
+      auto ar = eng->Align(rev.data(), len, gr, &sr);
+      if (sf >= sr) {
+        gr.AddAlignment(a, s, len);
+      } else {
+        gr.AddAlignment(ar, rev.data(), len);
+      }
+    }
+    std::string cons = gr.GenerateConsensus();
+    volatile std::size_t rows = gr.GenerateMultipleSequenceAlignment(false).size();
+    (void)rows;
+    if (cons.size() > cap) return -2;
+    std::memcpy(out, cons.data(), cons.size());
+    return static_cast<int>(cons.size());
+  } catch (std::exception& ex) {
+    g_error = ex.what();
+    return -1;
+  }
 }
 
 }  // extern "C"

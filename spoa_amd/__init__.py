@@ -48,7 +48,7 @@ class HipStats(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int32) for k in
-                ("type", "m", "n", "g", "e", "q", "c", "strand", "msa", "threads", "device")]
+                ("type", "m", "n", "g", "e", "q", "c", "strand", "msa", "threads", "device", "overlap")]
 
 
 class BatchStats(ctypes.Structure):
@@ -115,6 +115,11 @@ def lib():
     L.spoa_batch_context_free.argtypes = [ctypes.c_void_p]
     L.spoa_batch_context_device.restype = ctypes.c_void_p
     L.spoa_batch_context_device.argtypes = [ctypes.c_void_p]
+    L.spoa_batch_consensus_all.restype = ctypes.c_int64
+    L.spoa_batch_consensus_all.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_void_p]
+    L.spoa_batch_context_set_config.restype = ctypes.c_int
+    L.spoa_batch_context_set_config.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int32,
+                                                ctypes.c_int32]
     L.spoa_batch_context_run.restype = ctypes.c_void_p
     L.spoa_batch_context_run.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_char_p, ctypes.c_char_p]
@@ -318,29 +323,114 @@ class Windows:
                                   None if self.quals is None else [self.qualities(w) for w in idx])
 
 
+class _MsaView:
+    """Every window's MSA, read from the native results on first access of
+    each window (rows of equal width, as Graph::GenerateMultipleSequenceAlignment
+    returns them)."""
+
+    def __init__(self, owner, n, enabled):
+        self._owner = owner
+        self._n = n if enabled else 0
+        self._cache = {}
+
+    def __len__(self):
+        return self._n
+
+    def __getitem__(self, w):
+        if isinstance(w, slice):
+            return [self[i] for i in range(*w.indices(self._n))]
+        if w < 0:
+            w += self._n
+        if not 0 <= w < self._n:
+            raise IndexError(w)
+        if w not in self._cache:
+            self._cache[w] = self._owner._msa(w)
+        return self._cache[w]
+
+    def __iter__(self):
+        return (self[i] for i in range(self._n))
+
+    def __eq__(self, other):
+        return len(self) == len(other) and all(a == b for a, b in zip(self, other))
+
+
 class WindowResults:
-    def __init__(self, consensus, msa, reversed_, stats):
-        self.consensus = consensus
-        self.msa = msa
-        self.reversed = reversed_
-        self.stats = stats
+    """Results of a window run: consensus strings (a list), the MSAs (read
+    from the native results window by window, on access), the strand flags
+    and the run's measurements."""
+
+    def __init__(self, h, windows, msa, strand):
+        self._h = h
+        try:
+            self._load(windows, msa, strand)
+        except BaseException:
+            self._h = None
+            lib().spoa_batch_free(h)
+            raise
+
+    def _load(self, windows, msa, strand):
+        h = self._h
+        n = windows.n_windows
+        st = BatchStats()
+        lib().spoa_batch_get_stats(h, ctypes.byref(st))
+        self.stats = st.as_dict()
+        offs = np.zeros(n + 1, dtype=np.uint64)
+        cap = 1 << 20
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            k = lib().spoa_batch_consensus_all(h, buf, cap, offs.ctypes.data)
+            if k == -2:
+                cap = int(offs[n]) + 1
+                continue
+            break
+        raw = buf.raw[:k].decode()
+        o = offs.tolist()
+        self.consensus = [raw[o[w]:o[w + 1]] for w in range(n)]
+        self.msa = _MsaView(self, n, msa)
+        self.reversed = []
+        if strand:
+            for w in range(n):
+                nr = int(windows.win_read_off[w + 1] - windows.win_read_off[w])
+                rb = ctypes.create_string_buffer(max(1, nr))
+                k = lib().spoa_batch_reversed(h, w, rb, nr)
+                self.reversed.append(list(rb.raw[:k]))
+
+    def _msa(self, w):
+        rows, cols = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        cap = 1 << 16
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            rc = lib().spoa_batch_msa(self._h, w, buf, cap, ctypes.byref(rows), ctypes.byref(cols))
+            if rc == -2:
+                cap = rows.value * cols.value + 1
+                continue
+            break
+        cc = cols.value
+        raw = buf.raw
+        return [raw[i * cc:(i + 1) * cc].decode() for i in range(rows.value)]
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.spoa_batch_free(self._h)
+            self._h = None
 
 
-def _opts(type, m, n, g, e, q, c, strand, msa, threads, device):
+def _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap=False):
     if e is None:
         e = g
     if q is None:
         q, c = g, e
     return BatchOpts(type=type, m=m, n=n, g=g, e=e, q=q, c=c, strand=int(strand), msa=int(msa),
-                     threads=threads, device=device)
+                     threads=threads, device=device, overlap=int(overlap))
 
 
 class BatchContext:
     """A reusable batched-form engine (device buffers and host threads kept
     between runs) for one set of scoring options."""
 
-    def __init__(self, type, m, n, g, e=None, q=None, c=None, strand=False, msa=True, threads=0, device=0):
-        self.opts = _opts(type, m, n, g, e, q, c, strand, msa, threads, device)
+    def __init__(self, type, m, n, g, e=None, q=None, c=None, strand=False, msa=True, threads=0, device=0,
+                 overlap=False):
+        self.opts = _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap)
         self._h = lib().spoa_batch_context_new(ctypes.byref(self.opts))
         if not self._h:
             raise ValueError(_error())
@@ -353,6 +443,12 @@ class BatchContext:
     @property
     def device(self):
         return lib().spoa_batch_context_device(self._h)
+
+    def set_config(self, width_bits, cols_per_lane, waves, max_width=0xFFFFFFFF):
+        """Launch configuration of the batched launches for rows up to
+        max_width columns at one score width; cols_per_lane 0 clears."""
+        if lib().spoa_batch_context_set_config(self._h, width_bits, max_width, cols_per_lane, waves) != 0:
+            raise ValueError(_error())
 
     def run(self, windows):
         if not isinstance(windows, Windows):
@@ -381,41 +477,7 @@ def run_windows(windows, type, m, n, g, e=None, q=None, c=None, mode="batched", 
 def _collect(h, windows, msa, strand):
     if not h:
         raise ValueError(_error())
-    try:
-        st = BatchStats()
-        lib().spoa_batch_get_stats(h, ctypes.byref(st))
-        cons, msas, revs = [], [], []
-        for w in range(windows.n_windows):
-            cap = 1 << 16
-            while True:
-                buf = ctypes.create_string_buffer(cap)
-                k = lib().spoa_batch_consensus(h, w, buf, cap)
-                if k == -2:
-                    cap *= 4
-                    continue
-                break
-            cons.append(buf.raw[:k].decode())
-            if msa:
-                rows, cols = ctypes.c_uint32(0), ctypes.c_uint32(0)
-                cap = 1 << 16
-                while True:
-                    buf = ctypes.create_string_buffer(cap)
-                    rc = lib().spoa_batch_msa(h, w, buf, cap, ctypes.byref(rows), ctypes.byref(cols))
-                    if rc == -2:
-                        cap = rows.value * cols.value + 1
-                        continue
-                    break
-                cc = cols.value
-                raw = buf.raw
-                msas.append([raw[i * cc:(i + 1) * cc].decode() for i in range(rows.value)])
-            if strand:
-                nr = int(windows.win_read_off[w + 1] - windows.win_read_off[w])
-                rb = ctypes.create_string_buffer(max(1, nr))
-                k = lib().spoa_batch_reversed(h, w, rb, nr)
-                revs.append(list(rb.raw[:k]))
-        return WindowResults(cons, msas, revs, st.as_dict())
-    finally:
-        lib().spoa_batch_free(h)
+    return WindowResults(h, windows, msa, strand)
 
 
 def synth_windows(seed, n_windows, template=(500, 500), reads=(20, 20), read_len=(0, 0),

@@ -119,10 +119,18 @@ void ThreadPool::For(std::size_t n, const std::function<void(std::size_t)>& fn) 
 
 BatchContext::BatchContext(const BatchOptions& opt) : opt_(opt), pool_(opt.threads) {
   for (auto& e : engines_) e = spoa::AlignmentEngine::Create(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c);
-  // the halves share the device memory: each engine gets half the budget
-  for (auto& e : engines_) {
-    auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(e.get());
-    if (hip) spoa_hip_set_memory_budget(hip->device(), spoa_hip_memory_budget(hip->device()) / 2);
+  // the halves share the device memory: each engine gets half the budget;
+  // their kernels take turns on one compute stream (engines_[0]'s, which is
+  // destroyed last) while copies and host merging overlap
+  spoa_hip_engine* dev[2] = {nullptr, nullptr};
+  for (int h = 0; h < 2; ++h) {
+    auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(engines_[h].get());
+    if (!hip) throw std::invalid_argument("[spoa::hip] the batched form needs the HIP engine");
+    dev[h] = hip->device();
+    spoa_hip_set_memory_budget(dev[h], spoa_hip_memory_budget(dev[h]) / 2);
+  }
+  if (!opt.overlap && spoa_hip_share_compute(dev[1], dev[0]) != SPOA_HIP_OK) {
+    throw std::invalid_argument(spoa_hip_last_error());
   }
 }
 

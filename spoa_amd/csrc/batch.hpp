@@ -58,6 +58,7 @@ struct BatchOptions {
   std::int8_t m = 5, n = -4, g = -8, e = -6, q = -10, c = -4;
   bool strand = false;     // align both strands, keep the better (src/main.cpp:285-302)
   bool msa = true;         // also build the MSA of every window
+  bool overlap = false;    // the two halves' launches may share the device (else they take turns)
   unsigned threads = 0;    // host threads (0: all hardware threads)
   int device = 0;
 };

@@ -55,6 +55,7 @@ spoa_amd::BatchOptions Options(const spoa_batch_opts* o) {
   b.msa = o->msa != 0;
   b.threads = o->threads > 0 ? static_cast<unsigned>(o->threads) : 0u;
   b.device = o->device;
+  b.overlap = o->overlap != 0;
   return b;
 }
 
@@ -267,6 +268,20 @@ spoa_hip_engine* spoa_batch_context_device(void* ctx) {
   return hip ? hip->device() : nullptr;
 }
 
+int spoa_batch_context_set_config(void* ctx, int32_t width_bits, uint32_t max_width, int32_t cols_per_lane,
+                                  int32_t waves) {
+  auto* c = static_cast<spoa_amd::BatchContext*>(ctx);
+  for (int h = 0; h < 2; ++h) {
+    auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(c->engine(h));
+    if (!hip) return -1;
+    if (spoa_hip_set_config(hip->device(), width_bits, 1, max_width, cols_per_lane, waves) != SPOA_HIP_OK) {
+      g_error = spoa_hip_last_error();
+      return -1;
+    }
+  }
+  return 0;
+}
+
 void* spoa_batch_context_run(void* ctx, uint32_t n_windows, const uint32_t* win_read_off, const uint64_t* read_off,
                              const char* seqs, const char* quals) {
   try {
@@ -313,6 +328,21 @@ int spoa_batch_consensus(void* h, uint32_t w, char* out, uint32_t cap) {
   if (s.size() > cap) return -2;
   std::memcpy(out, s.data(), s.size());
   return static_cast<int>(s.size());
+}
+
+int64_t spoa_batch_consensus_all(void* h, char* out, uint64_t cap, uint64_t* offs) {
+  const auto& r = static_cast<BatchHandle*>(h)->results;
+  uint64_t total = 0;
+  for (std::size_t w = 0; w < r.size(); ++w) {
+    offs[w] = total;
+    total += r[w].consensus.size();
+  }
+  offs[r.size()] = total;
+  if (total > cap) return -2;
+  for (std::size_t w = 0; w < r.size(); ++w) {
+    std::memcpy(out + offs[w], r[w].consensus.data(), r[w].consensus.size());
+  }
+  return static_cast<int64_t>(total);
 }
 
 int spoa_batch_msa(void* h, uint32_t w, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols) {

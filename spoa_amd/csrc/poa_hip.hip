@@ -207,6 +207,7 @@ KParams MakeParams(const spoa_hip_params& p, int width_bits, int k) {
     const int L = Lanes(width_bits);
     const int32_t seg[4] = {(L - 1) * p.e + std::max<int32_t>(p.g, p.e), (L - 1) * p.c + p.g,
                             (L - 1) * p.e + p.q, (L - 1) * p.c + std::max<int32_t>(p.q, p.c)};
+    std::memcpy(kp.mseg, seg, sizeof(seg));
     int32_t lane[4];
     std::memcpy(lane, seg, sizeof(seg));
     for (int s = 1; s < k / L; ++s) MpMul(seg, lane, lane);
@@ -266,14 +267,17 @@ struct Plan {
   uint64_t seq_bytes;
 };
 
-constexpr int kStreams = 4;
-
 struct spoa_hip_engine {
   int device = 0;
-  hipStream_t stream = nullptr;            // copies, counters and the first group
-  hipStream_t side[kStreams - 1] = {};     // further launch groups of a pass
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
-  hipEvent_t ev_group[kStreams] = {};
+  hipStream_t stream = nullptr;   // copies
+  // kernels: the engine's own stream, or one shared with other engines so
+  // that their launches run one after another and never share the device
+  hipStream_t compute = nullptr;
+  hipStream_t compute_own = nullptr;  // created when other engines share this one's kernels
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;  // copy timing
+  hipEvent_t ev_in = nullptr, ev_kdone = nullptr;  // copy -> kernels -> copy ordering
+  std::vector<hipEvent_t> ev_ks, ev_ke;            // start / end of every launch of a pass
+  size_t pass_launches = 0;
   spoa_hip_params prm{};
   int num_cus = 256;
   uint64_t budget = 0;
@@ -301,26 +305,17 @@ struct spoa_hip_engine {
 
 namespace {
 
-// Batched launches favour one wave per job (rows up to 64 * 32 columns),
-// the per-call launch the widest workgroup (up to 16 waves) of short lanes.
+// Batched launches: 16 columns a lane for int16 rows (at most 1,024
+// columns a wave), 8 for int32 rows (512 a wave), as many waves as the row
+// needs up to 8 (wider rows take several tiles).  Measured on cfg-2/cfg-3
+// shaped batches, this beats both longer lanes (K = 32, one wave) and more
+// waves of shorter lanes.  The per-call launch takes the widest workgroup
+// (up to 8 waves) of the shortest lanes.
 Cfg DefaultCfg(const spoa_hip_engine* E, uint32_t W, int width_bits, bool batch) {
   Cfg c;
   const bool convex16 = E->prm.subtype == 2 && width_bits == 16;
   const int kmin = convex16 ? 16 : 8;
-  // int32 rows hold twice the registers of int16 rows: at most 16 columns a lane
-  const int kmax = width_bits == 32 ? 16 : 32;
-  if (batch) {
-    for (int k = kmin; k <= kmax; k *= 2) {
-      if (W <= 64u * static_cast<uint32_t>(k)) {
-        c.k = k;
-        c.waves = 1;
-        return c;
-      }
-    }
-    c.k = 16;
-  } else {
-    c.k = kmin;
-  }
+  c.k = batch ? (width_bits == 16 ? 16 : 8) : kmin;
   const uint32_t per_wave = 64u * static_cast<uint32_t>(c.k);
   c.waves = static_cast<int>(
       std::min<uint32_t>(kMaxGroupWaves, std::max<uint32_t>(1u, (W + per_wave - 1) / per_wave)));
@@ -398,9 +393,9 @@ bool SameGroup(const Plan& a, const Plan& b) {
 
 // One pass: the jobs plans[lo, hi) (sorted by launch group, largest first
 // inside a group) are packed into one staging block, copied in one
-// transfer, aligned by one persistent launch per group -- the groups on
-// separate streams, so they share the device -- and copied back in one
-// transfer.
+// transfer on the engine's copy stream, aligned by one persistent launch
+// per group on the compute stream (one after another, each with the device
+// to itself and timed by its own events) and copied back in one transfer.
 int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& plans_in, size_t lo, size_t hi) {
   auto& pd = E->pending;
   pd.plans = std::move(plans_in);
@@ -408,7 +403,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   const spoa_hip_params& P = E->prm;
   const uint32_t nj = static_cast<uint32_t>(hi - lo);
 
-  uint64_t graph_words = 0, seq_bytes = 0, mat_bytes = 0, pairs = 0;
+  uint64_t graph_words = 0, seq_bytes = 0, pairs = 0;
   std::vector<uint64_t> g_at(nj), s_at(nj), p_at(nj);
   for (size_t x = lo; x < hi; ++x) {
     g_at[x - lo] = graph_words;
@@ -416,7 +411,6 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     p_at[x - lo] = pairs;
     graph_words += plans[x].graph_words;
     seq_bytes += plans[x].seq_bytes;
-    mat_bytes += plans[x].mat_bytes;
     pairs += jobs[plans[x].job].graph.num_nodes + jobs[plans[x].job].len;
   }
   const uint64_t off_jobs = 0;
@@ -560,25 +554,36 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   }
   if (groups.size() > 256) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] too many launch groups");
 
+  while (E->ev_ks.size() < groups.size()) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (!HipOk(hipEventCreate(&a), "hipEventCreate")) return SPOA_HIP_DEVICE_ERROR;
+    if (!HipOk(hipEventCreate(&b), "hipEventCreate")) {
+      (void)hipEventDestroy(a);
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+    E->ev_ks.push_back(a);
+    E->ev_ke.push_back(b);
+  }
+
   uint8_t* din = static_cast<uint8_t*>(E->d_in.p);
   uint8_t* dout = static_cast<uint8_t*>(E->d_out.p);
   uint32_t* counters = static_cast<uint32_t*>(E->d_counter.p);
   hipStream_t st = E->stream;
+  hipStream_t cs = E->compute;
   if (!HipOk(hipEventRecord(E->ev0, st), "event") ||
-      !HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in")) {
+      !HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in") ||
+      !HipOk(hipEventRecord(E->ev1, st), "event")) {
     return SPOA_HIP_DEVICE_ERROR;
   }
-  spoa_hip_dev::ResetCounters<<<1, 256, 0, st>>>(counters, static_cast<uint32_t>(groups.size()));
-  if (!HipOk(hipGetLastError(), "counter reset") || !HipOk(hipEventRecord(E->ev1, st), "event")) {
+  if (cs != st && (!HipOk(hipEventRecord(E->ev_in, st), "event") ||
+                   !HipOk(hipStreamWaitEvent(cs, E->ev_in, 0), "stream wait"))) {
     return SPOA_HIP_DEVICE_ERROR;
   }
+  spoa_hip_dev::ResetCounters<<<1, 256, 0, cs>>>(counters, static_cast<uint32_t>(groups.size()));
+  if (!HipOk(hipGetLastError(), "counter reset")) return SPOA_HIP_DEVICE_ERROR;
+  // the groups run one after another on the compute stream, each timed
   for (size_t gi = 0; gi < groups.size(); ++gi) {
     const Group& gp = groups[gi];
-    const int si = static_cast<int>(gi % kStreams);
-    hipStream_t gs = si == 0 ? st : E->side[si - 1];
-    if (si != 0 && gi < kStreams && !HipOk(hipStreamWaitEvent(gs, E->ev1, 0), "stream wait")) {
-      return SPOA_HIP_DEVICE_ERROR;
-    }
     KArgs args{};
     args.prm = gp.prm;
     args.jobs = reinterpret_cast<const DevJob*>(din + off_jobs) + gp.a;
@@ -588,16 +593,17 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     args.seqs = din + off_seq;
     args.pairs = reinterpret_cast<int32_t*>(dout + off_pairs);
     args.results = reinterpret_cast<int32_t*>(dout + off_res) + 4 * gp.a;
-    gp.fn<<<gp.grid, gp.threads, gp.lds, gs>>>(args);
-    if (!HipOk(hipGetLastError(), "kernel launch")) return SPOA_HIP_DEVICE_ERROR;
-    E->stats.launches += 1;
-  }
-  // join the side streams
-  for (int si = 1; si < kStreams && si < static_cast<int>(groups.size()); ++si) {
-    if (!HipOk(hipEventRecord(E->ev_group[si], E->side[si - 1]), "event") ||
-        !HipOk(hipStreamWaitEvent(st, E->ev_group[si], 0), "stream wait")) {
+    if (!HipOk(hipEventRecord(E->ev_ks[gi], cs), "event")) return SPOA_HIP_DEVICE_ERROR;
+    gp.fn<<<gp.grid, gp.threads, gp.lds, cs>>>(args);
+    if (!HipOk(hipGetLastError(), "kernel launch") || !HipOk(hipEventRecord(E->ev_ke[gi], cs), "event")) {
       return SPOA_HIP_DEVICE_ERROR;
     }
+    E->stats.launches += 1;
+  }
+  E->pass_launches = groups.size();
+  if (cs != st && (!HipOk(hipEventRecord(E->ev_kdone, cs), "event") ||
+                   !HipOk(hipStreamWaitEvent(st, E->ev_kdone, 0), "stream wait"))) {
+    return SPOA_HIP_DEVICE_ERROR;
   }
   if (!HipOk(hipEventRecord(E->ev2, st), "event") ||
       !HipOk(hipMemcpyAsync(E->h_out.p, E->d_out.p, out_bytes, hipMemcpyDeviceToHost, st), "copy out") ||
@@ -627,13 +633,16 @@ int RunPassFinish(spoa_hip_engine* E) {
   if (!HipOk(hipStreamSynchronize(st), "kernel")) return SPOA_HIP_DEVICE_ERROR;
   const auto t_wait1 = std::chrono::steady_clock::now();
   E->stats.wait_ms += std::chrono::duration<double, std::milli>(t_wait1 - t_wait0).count();
-  float ms_in = 0, ms_k = 0, ms_out = 0;
+  float ms_in = 0, ms_out = 0;
   (void)hipEventElapsedTime(&ms_in, E->ev0, E->ev1);
-  (void)hipEventElapsedTime(&ms_k, E->ev1, E->ev2);
   (void)hipEventElapsedTime(&ms_out, E->ev2, E->ev3);
   E->stats.h2d_ms += ms_in;
-  E->stats.kernel_ms += ms_k;
   E->stats.d2h_ms += ms_out;
+  for (size_t gi = 0; gi < E->pass_launches; ++gi) {
+    float ms_k = 0;
+    (void)hipEventElapsedTime(&ms_k, E->ev_ks[gi], E->ev_ke[gi]);
+    E->stats.kernel_ms += ms_k;
+  }
 
   // ---- unpack: pairs come back in traceback order
   const int32_t* res = E->h_out.At<int32_t>(pd.off_res);
@@ -689,6 +698,20 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
       g_last_error = msg;
     }
   };
+  // A batch runs one launch configuration per score width, the one for its
+  // longest read at that width: jobs of one width then share one launch
+  // (shorter rows leave lanes idle) instead of several launches in turn.
+  uint32_t cfg_len[2] = {0, 0};
+  if (batch) {
+    for (uint32_t x = 0; x < n_jobs; ++x) {
+      const spoa_hip_job& j = jobs[x];
+      if (j.graph.num_nodes == 0 || j.len == 0 || j.len > static_cast<uint32_t>(std::numeric_limits<int32_t>::max())) {
+        continue;
+      }
+      const int width = WidthBits(P, j.len, j.graph.num_nodes);
+      if (width != 0) cfg_len[width == 32] = std::max(cfg_len[width == 32], j.len);
+    }
+  }
   std::vector<Plan> plans;
   plans.reserve(n_jobs);
   for (uint32_t x = 0; x < n_jobs; ++x) {
@@ -712,7 +735,7 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     Plan pl;
     pl.job = x;
     pl.width = width;
-    pl.cfg = ChooseCfg(E, j.len, width, batch);
+    pl.cfg = ChooseCfg(E, batch ? cfg_len[width == 32] : j.len, width, batch);
     const int L = Lanes(width);
     pl.W = (j.len + L - 1) / L * L;
     pl.stride = static_cast<uint32_t>(RoundUp(pl.W, static_cast<uint64_t>(pl.cfg.k)));
@@ -799,22 +822,13 @@ int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_e
   }
   if (!HipOk(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking), "hipStreamCreate") ||
       !HipOk(hipEventCreate(&E->ev0), "hipEventCreate") || !HipOk(hipEventCreate(&E->ev1), "hipEventCreate") ||
-      !HipOk(hipEventCreate(&E->ev2), "hipEventCreate") || !HipOk(hipEventCreate(&E->ev3), "hipEventCreate")) {
+      !HipOk(hipEventCreate(&E->ev2), "hipEventCreate") || !HipOk(hipEventCreate(&E->ev3), "hipEventCreate") ||
+      !HipOk(hipEventCreateWithFlags(&E->ev_in, hipEventDisableTiming), "hipEventCreate") ||
+      !HipOk(hipEventCreateWithFlags(&E->ev_kdone, hipEventDisableTiming), "hipEventCreate")) {
     spoa_hip_engine_destroy(E);
     return SPOA_HIP_DEVICE_ERROR;
   }
-  for (int i = 0; i < kStreams - 1; ++i) {
-    if (!HipOk(hipStreamCreateWithFlags(&E->side[i], hipStreamNonBlocking), "hipStreamCreate")) {
-      spoa_hip_engine_destroy(E);
-      return SPOA_HIP_DEVICE_ERROR;
-    }
-  }
-  for (int i = 0; i < kStreams; ++i) {
-    if (!HipOk(hipEventCreateWithFlags(&E->ev_group[i], hipEventDisableTiming), "hipEventCreate")) {
-      spoa_hip_engine_destroy(E);
-      return SPOA_HIP_DEVICE_ERROR;
-    }
-  }
+  E->compute = E->stream;
   *out = E;
   return SPOA_HIP_OK;
 }
@@ -822,6 +836,7 @@ int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_e
 int spoa_hip_engine_destroy(spoa_hip_engine* E) {
   if (!E) return SPOA_HIP_OK;
   (void)hipSetDevice(E->device);
+  if (E->pending.active) (void)RunPassFinish(E);
   if (E->stream) (void)hipStreamSynchronize(E->stream);
   E->d_in.Release();
   E->d_out.Release();
@@ -834,11 +849,13 @@ int spoa_hip_engine_destroy(spoa_hip_engine* E) {
   if (E->ev1) (void)hipEventDestroy(E->ev1);
   if (E->ev2) (void)hipEventDestroy(E->ev2);
   if (E->ev3) (void)hipEventDestroy(E->ev3);
-  for (int i = 0; i < kStreams - 1; ++i) {
-    if (E->side[i]) (void)hipStreamDestroy(E->side[i]);
-  }
-  for (int i = 0; i < kStreams; ++i) {
-    if (E->ev_group[i]) (void)hipEventDestroy(E->ev_group[i]);
+  if (E->ev_in) (void)hipEventDestroy(E->ev_in);
+  if (E->ev_kdone) (void)hipEventDestroy(E->ev_kdone);
+  for (auto ev : E->ev_ks) (void)hipEventDestroy(ev);
+  for (auto ev : E->ev_ke) (void)hipEventDestroy(ev);
+  if (E->compute_own) {
+    (void)hipStreamSynchronize(E->compute_own);
+    (void)hipStreamDestroy(E->compute_own);
   }
   if (E->stream) (void)hipStreamDestroy(E->stream);
   delete E;
@@ -952,6 +969,26 @@ int spoa_hip_set_config(spoa_hip_engine* E, int32_t width_bits, int32_t batch, u
   c.k = k;
   c.waves = waves;
   table[max_width] = c;
+  return SPOA_HIP_OK;
+}
+
+int spoa_hip_share_compute(spoa_hip_engine* E, spoa_hip_engine* owner) {
+  if (!E) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null engine");
+  if (owner && owner->device != E->device) {
+    return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] engines on different devices cannot share a stream");
+  }
+  if (E->pending.active) (void)RunPassFinish(E);
+  if (owner && owner->compute == owner->stream) {
+    // the owner's kernels move off its copy stream, so no engine's copies
+    // wait behind another engine's kernels
+    if (owner->pending.active) (void)RunPassFinish(owner);
+    if (!HipOk(hipSetDevice(owner->device), "hipSetDevice") ||
+        !HipOk(hipStreamCreateWithFlags(&owner->compute_own, hipStreamNonBlocking), "hipStreamCreate")) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+    owner->compute = owner->compute_own;
+  }
+  E->compute = owner ? owner->compute : E->stream;
   return SPOA_HIP_OK;
 }
 

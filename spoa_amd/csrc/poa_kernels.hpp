@@ -49,6 +49,7 @@ struct KParams {
   int32_t ninf;     // the score type's minus infinity (row 0 of F and O)
   int32_t decay;    // linear: K*g; affine: K*e (per-lane decay of the scan)
   int32_t mpow[8][4];  // convex: M^(2^t) for t = 0..6 (row-major 2x2), [7] unused
+  int32_t mseg[4];     // convex: M of one L-column segment
 };
 
 // One (graph, read) job.  Offsets are in elements of the respective buffer.
