@@ -120,6 +120,44 @@ __device__ __forceinline__ void MpApply(const int32_t* M, int32_t& y0, int32_t& 
   y1 = b;
 }
 
+// out <- M^k (k >= 1) from the powers M^(2^t) (max-plus; powers of one
+// matrix commute).
+__device__ __forceinline__ void LanePow(const KParams& prm, int k, int32_t (&out)[4]) {
+  bool have = false;
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    if ((k >> t) & 1) {
+      const int32_t* b = prm.mpow[t];
+      if (!have) {
+        out[0] = b[0];
+        out[1] = b[1];
+        out[2] = b[2];
+        out[3] = b[3];
+        have = true;
+      } else {
+        const int32_t r0 = max(out[0] + b[0], out[1] + b[2]);
+        const int32_t r1 = max(out[0] + b[1], out[1] + b[3]);
+        const int32_t r2 = max(out[2] + b[0], out[3] + b[2]);
+        const int32_t r3 = max(out[2] + b[1], out[3] + b[3]);
+        out[0] = r0;
+        out[1] = r1;
+        out[2] = r2;
+        out[3] = r3;
+      }
+    }
+  }
+}
+
+// DPP lane moves (gfx9 encodings): row_shr:n within rows of 16 lanes,
+// wave_shr:1 across the wave.  Lanes whose source is out of range keep old.
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kWaveShr1 = 0x138;
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int32_t Dpp(int32_t old, int32_t v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
+}
+
 __device__ __forceinline__ uint32_t Uniform(uint32_t v) {
   return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
 }
@@ -343,6 +381,41 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
 // wrap-around is the reference's int16 SIMD arithmetic.
 typedef short S16x2 __attribute__((ext_vector_type(2)));
 
+__device__ __forceinline__ S16x2 AsPk(uint32_t x) { return __builtin_bit_cast(S16x2, x); }
+__device__ __forceinline__ uint32_t AsU(S16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ uint32_t PkAdd(uint32_t a, uint32_t b) { return AsU(AsPk(a) + AsPk(b)); }
+// saturating: a carry term far below every real score stays at -32768
+// instead of wrapping round
+__device__ __forceinline__ uint32_t PkAddSat(uint32_t a, uint32_t b) {
+  return AsU(__builtin_elementwise_add_sat(AsPk(a), AsPk(b)));
+}
+__device__ __forceinline__ uint32_t PkMax(uint32_t a, uint32_t b) {
+  return AsU(__builtin_elementwise_max(AsPk(a), AsPk(b)));
+}
+__device__ __forceinline__ uint32_t DupLo(uint32_t a) {
+  const S16x2 v = AsPk(a);
+  return AsU(S16x2{v.x, v.x});
+}
+__device__ __forceinline__ uint32_t DupHi(uint32_t a) {
+  const S16x2 v = AsPk(a);
+  return AsU(S16x2{v.y, v.y});
+}
+__device__ __forceinline__ uint32_t SwapHalves(uint32_t a) {
+  const S16x2 v = AsPk(a);
+  return AsU(S16x2{v.y, v.x});
+}
+__device__ __forceinline__ uint32_t Pack16(int32_t lo, int32_t hi) {
+  return (static_cast<uint32_t>(lo) & 0xffffu) | (static_cast<uint32_t>(hi) << 16);
+}
+__device__ __forceinline__ int32_t Lo16(uint32_t w) { return static_cast<int32_t>(static_cast<int16_t>(w)); }
+__device__ __forceinline__ int32_t Hi16(uint32_t w) { return static_cast<int32_t>(w) >> 16; }
+__device__ __forceinline__ int32_t Clamp16(int32_t v) { return min(max(v, -32768), 32767); }
+// column k of packed words, in both halves
+template <int KW>
+__device__ __forceinline__ uint32_t HDup(const uint32_t (&w)[KW], int k) {
+  return (k & 1) ? DupHi(w[k >> 1]) : DupLo(w[k >> 1]);
+}
+
 template <typename ST>
 struct Cols {
   static constexpr int kPerWord = sizeof(ST) == 2 ? 2 : 1;
@@ -476,7 +549,6 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
   __shared__ int32_t s_tot[2][kMaxWaves][2];
   __shared__ int32_t s_red[kMaxWaves][3];
   __shared__ uint32_t s_job;
-  __shared__ int32_t s_mpow[7][4];  // convex scan matrices (kept out of scalar registers)
   extern __shared__ uint32_t s_prof[];
 
   const KParams& prm = a.prm;
@@ -489,7 +561,18 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
   const int32_t m = prm.m, mis = prm.mis, g = prm.g, e = prm.e, q = prm.q, cc = prm.c;
   const int32_t pad = prm.pad;
   const uint32_t G2 = Splat<ST>(g), E2 = Splat<ST>(e), Q2 = Splat<ST>(q), C2 = Splat<ST>(cc);
-  if (SUB == 2 && tid < 28) s_mpow[tid / 4][tid % 4] = prm.mpow[tid / 4][tid % 4];
+  // scan constants: the transfer across the lanes of this lane's row up to
+  // it (the carry out of the previous row of 16 lanes reaches it), and, for
+  // multi-wave rows, from lane 0 to it (the wave carry)
+  int32_t lda = 0, ldc = 0;
+  int32_t lpa[4] = {0, 0, 0, 0}, lpc[4] = {0, 0, 0, 0};
+  if constexpr (SUB != 2) {
+    lda = ((lane & 15) + 1) * prm.decay;
+    if constexpr (MAXT > 64) ldc = lane * prm.decay;
+  } else {
+    LanePow(prm, (lane & 15) + 1, lpa);
+    if constexpr (MAXT > 64) LanePow(prm, lane >= 1 ? lane : 1, lpc);
+  }
 
   for (;;) {
     if (tid == 0) s_job = atomicAdd(a.queue, 1u);
@@ -752,66 +835,169 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
           }
         }
 
-        // ---- 2. lane transfer with the carry absent
+        // ---- 2. local pass: the horizontal gaps inside the lane with the
+        // carry absent.  loc holds, per column, the best horizontal value
+        // reachable from inside the lane (linear H, affine E, convex (E, Q)),
+        // and (b0, b1) what the lane hands its right neighbour.  Column c0
+        // itself takes its value from the carry alone (loc[0] is unused
+        // there for affine / convex).
+        constexpr int NLOC = (SUB == 2 && CPW == 2) ? K : KW;
+        uint32_t loc[NLOC];
+        uint32_t qloc[(SUB == 2 && CPW == 1) ? K : 1];
         int32_t b0 = 0, b1 = 0;
-        if (SUB == 0) {
-          int32_t x = Col<ST, KW>(h1, 0);
+        if constexpr (CPW == 2) {
+          if constexpr (SUB == 0) {
+            // word i = columns (2i, 2i+1): (a, max(b, a + g)), then the chain
+            const uint32_t k0g = Pack16(0, g), kgg = Pack16(g, 2 * g);
 #pragma unroll
-          for (int k = 1; k < K; ++k) x = max(Col<ST, KW>(h1, k), x + g);
-          b0 = x + g;
-        } else if (SUB == 1) {
-          int32_t x = Col<ST, KW>(h1, 0) + g;  // E at column c0 + 1
+            for (int i = 0; i < KW; ++i) {
+              const uint32_t wp = PkMax(h1[i], PkAdd(DupLo(h1[i]), k0g));
+              loc[i] = i == 0 ? wp : PkMax(wp, PkAdd(DupHi(loc[i - 1]), kgg));
+            }
+            b0 = Hi16(loc[KW - 1]) + g;
+          } else if constexpr (SUB == 1) {
+            // E[c] = max(H1[c-1] + g, E[c-1] + e) on (c-1, c) pairs
+            const uint32_t k0e = Pack16(0, e), kee = Pack16(e, 2 * e);
 #pragma unroll
-          for (int k = 2; k < K; ++k) x = max(Col<ST, KW>(h1, k - 1) + g, x + e);
-          b0 = max(Col<ST, KW>(h1, K - 1) + g, x + e);
+            for (int i = 0; i < KW; ++i) {
+              const uint32_t sh = i == 0 ? (h1[0] << 16) : __builtin_amdgcn_alignbit(h1[i], h1[i - 1], 16);
+              const uint32_t gi = PkAdd(sh, G2);  // (H1[2i-1] + g, H1[2i] + g)
+              if (i == 0) {
+                loc[0] = gi;  // high half: E at column c0 + 1
+              } else {
+                const uint32_t a2 = PkMax(gi, PkAdd(DupLo(gi), k0e));
+                loc[i] = PkMax(a2, PkAdd(DupHi(loc[i - 1]), kee));
+              }
+            }
+            b0 = max(Hi16(h1[KW - 1]) + g, Hi16(loc[KW - 1]) + e);
+          } else {
+            // (E, Q) of every column in one word; a gap opens from H1
+            // inside an L-segment and from the final H at a segment start
+            const uint32_t GQ = Pack16(g, q), EC = Pack16(e, cc);
+            loc[0] = 0u;
+            loc[1] = PkAdd(HDup(h1, 0), GQ);
+#pragma unroll
+            for (int k = 2; k < K; ++k) {
+              if (k % L == 0) {
+                const uint32_t hf = PkMax(PkMax(loc[k - 1], SwapHalves(loc[k - 1])), HDup(h1, k - 1));
+                loc[k] = PkMax(PkAdd(hf, GQ), PkAdd(loc[k - 1], EC));
+              } else {
+                loc[k] = PkMax(PkAdd(HDup(h1, k - 1), GQ), PkAdd(loc[k - 1], EC));
+              }
+            }
+            const uint32_t hf = PkMax(PkMax(loc[K - 1], SwapHalves(loc[K - 1])), HDup(h1, K - 1));
+            const uint32_t nb = PkMax(PkAdd(hf, GQ), PkAdd(loc[K - 1], EC));
+            b0 = Lo16(nb);
+            b1 = Hi16(nb);
+          }
         } else {
-          int32_t xe = Col<ST, KW>(h1, 0) + g, xq = Col<ST, KW>(h1, 0) + q;  // E, Q at column c0 + 1
+          auto hk = [&](int k) -> int32_t { return static_cast<int32_t>(h1[k]); };
+          if constexpr (SUB == 0) {
+            int32_t x = hk(0);
+            loc[0] = static_cast<uint32_t>(x);
 #pragma unroll
-          for (int k = 2; k < K; ++k) {
-            const int32_t hp = Col<ST, KW>(h1, k - 1);
-            if (k % L == 0) {
-              const int32_t hf = max(hp, max(xe, xq));
-              const int32_t ne = max(hf + g, xe + e);
-              const int32_t nq = max(hf + q, xq + cc);
+            for (int k = 1; k < K; ++k) {
+              x = max(hk(k), x + g);
+              loc[k] = static_cast<uint32_t>(x);
+            }
+            b0 = x + g;
+          } else if constexpr (SUB == 1) {
+            int32_t x = hk(0) + g;
+            loc[0] = 0u;
+            loc[1] = static_cast<uint32_t>(x);
+#pragma unroll
+            for (int k = 2; k < K; ++k) {
+              x = max(hk(k - 1) + g, x + e);
+              loc[k] = static_cast<uint32_t>(x);
+            }
+            b0 = max(hk(K - 1) + g, x + e);
+          } else {
+            int32_t xe = hk(0) + g, xq = hk(0) + q;
+            loc[0] = 0u;
+            qloc[0] = 0u;
+            loc[1] = static_cast<uint32_t>(xe);
+            qloc[1] = static_cast<uint32_t>(xq);
+#pragma unroll
+            for (int k = 2; k < K; ++k) {
+              const int32_t hp = hk(k - 1);
+              const int32_t xs = k % L == 0 ? max(hp, max(xe, xq)) : hp;
+              const int32_t ne = max(xs + g, xe + e);
+              const int32_t nq = max(xs + q, xq + cc);
               xe = ne;
               xq = nq;
-            } else {
-              xe = max(hp + g, xe + e);
-              xq = max(hp + q, xq + cc);
+              loc[k] = static_cast<uint32_t>(xe);
+              qloc[k] = static_cast<uint32_t>(xq);
             }
+            const int32_t hf = max(hk(K - 1), max(xe, xq));
+            b0 = max(hf + g, xe + e);
+            b1 = max(hf + q, xq + cc);
           }
-          const int32_t hf = max(Col<ST, KW>(h1, K - 1), max(xe, xq));
-          b0 = max(hf + g, xe + e);
-          b1 = max(hf + q, xq + cc);
         }
 
-        // ---- wavefront inclusive scan of the transfers
+        // ---- wavefront inclusive scan of the lane transfers.  Inside each
+        // row of 16 lanes a Kogge-Stone scan on DPP row shifts, the sender
+        // applying the uniform step transfer M^sh (scalar: sh times the lane
+        // decay); across rows, the inclusive values at the ends of rows 0-2
+        // (read into scalar registers) are chained with M^16 and moved into
+        // the next row by each lane's own M^(lane % 16 + 1).  A one-wave row
+        // folds the tile carry into lane 0 first, so the exclusive carry of
+        // every lane is its left neighbour's inclusive value.
+        const bool one_wave = nwaves == 1;
         int32_t s0 = b0, s1 = b1;
-        if (SUB != 2) {
-          const int32_t d = prm.decay;
-#pragma unroll
-          for (int st = 0; st < 6; ++st) {
-            const int sh = 1 << st;
-            const int32_t y = __shfl_up(s0, sh, 64);
-            if (lane >= sh) s0 = max(s0, y + sh * d);
+        if (one_wave && lane == 0) {
+          if constexpr (SUB != 2) {
+            s0 = max(s0, y0 + prm.decay);
+          } else {
+            int32_t u0 = y0, u1 = y1;
+            MpApply(prm.mpow[0], u0, u1);
+            s0 = max(s0, u0);
+            s1 = max(s1, u1);
           }
+        }
+        if constexpr (SUB != 2) {
+          const int32_t d = prm.decay;
+          s0 = max(s0, Dpp<kRowShr1, 0xf>(s0, s0 + d));
+          s0 = max(s0, Dpp<kRowShr2, 0xf>(s0, s0 + 2 * d));
+          s0 = max(s0, Dpp<kRowShr4, 0xf>(s0, s0 + 4 * d));
+          s0 = max(s0, Dpp<kRowShr8, 0xf>(s0, s0 + 8 * d));
+          const int32_t r1 = __builtin_amdgcn_readlane(s0, 15);
+          const int32_t r2 = max(__builtin_amdgcn_readlane(s0, 31), r1 + 16 * d);
+          const int32_t r3 = max(__builtin_amdgcn_readlane(s0, 47), r2 + 16 * d);
+          const int32_t rr = lane < 32 ? r1 : (lane < 48 ? r2 : r3);
+          if (lane >= 16) s0 = max(s0, rr + lda);
         } else {
-#pragma unroll
-          for (int st = 0; st < 6; ++st) {
-            const int sh = 1 << st;
-            int32_t z0 = __shfl_up(s0, sh, 64);
-            int32_t z1 = __shfl_up(s1, sh, 64);
-            if (lane >= sh) {
-              MpApply(s_mpow[st], z0, z1);
-              s0 = max(s0, z0);
-              s1 = max(s1, z1);
-            }
+#define SPOA_ROW_STEP(CTRL, T)                                               \
+  {                                                                          \
+    const int32_t u0 = max(prm.mpow[T][0] + s0, prm.mpow[T][1] + s1);        \
+    const int32_t u1 = max(prm.mpow[T][2] + s0, prm.mpow[T][3] + s1);        \
+    s0 = max(s0, Dpp<CTRL, 0xf>(s0, u0));                                    \
+    s1 = max(s1, Dpp<CTRL, 0xf>(s1, u1));                                    \
+  }
+          SPOA_ROW_STEP(kRowShr1, 0)
+          SPOA_ROW_STEP(kRowShr2, 1)
+          SPOA_ROW_STEP(kRowShr4, 2)
+          SPOA_ROW_STEP(kRowShr8, 3)
+#undef SPOA_ROW_STEP
+          int32_t a0 = __builtin_amdgcn_readlane(s0, 15), a1 = __builtin_amdgcn_readlane(s1, 15);
+          int32_t c0r = a0, c1r = a1;  // carry out of row 0
+          MpApply(prm.mpow[4], a0, a1);
+          a0 = max(a0, __builtin_amdgcn_readlane(s0, 31));
+          a1 = max(a1, __builtin_amdgcn_readlane(s1, 31));
+          int32_t c0s = a0, c1s = a1;  // out of rows 0-1
+          MpApply(prm.mpow[4], a0, a1);
+          a0 = max(a0, __builtin_amdgcn_readlane(s0, 47));
+          a1 = max(a1, __builtin_amdgcn_readlane(s1, 47));  // out of rows 0-2
+          const int32_t z0 = lane < 32 ? c0r : (lane < 48 ? c0s : a0);
+          const int32_t z1 = lane < 32 ? c1r : (lane < 48 ? c1s : a1);
+          if (lane >= 16) {
+            s0 = max(s0, max(lpa[0] + z0, lpa[1] + z1));
+            s1 = max(s1, max(lpa[2] + z0, lpa[3] + z1));
           }
         }
         // carry into this wave: the tile carry folded with earlier waves
         int32_t w0 = y0, w1 = y1;
         int32_t tile_out0, tile_out1;
-        if (nwaves > 1) {
+        if (!one_wave) {
           if (lane == 63) {
             s_tot[par][wave][0] = s0;
             s_tot[par][wave][1] = s1;
@@ -827,7 +1013,7 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
             if (SUB != 2) {
               f0 = max(t0, f0 + 64 * prm.decay);
             } else {
-              MpApply(s_mpow[6], f0, f1);
+              MpApply(prm.mpow[6], f0, f1);
               f0 = max(f0, t0);
               f1 = max(f1, t1);
             }
@@ -836,101 +1022,144 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
           tile_out1 = f1;
           par ^= 1;
         } else {
-          const int32_t t0 = __shfl(s0, 63, 64), t1 = __shfl(s1, 63, 64);
-          int32_t f0 = y0, f1 = y1;
-          if (SUB != 2) {
-            f0 = max(t0, f0 + 64 * prm.decay);
-          } else {
-            MpApply(s_mpow[6], f0, f1);
-            f0 = max(f0, t0);
-            f1 = max(f1, t1);
-          }
-          tile_out0 = f0;
-          tile_out1 = f1;
+          // lane 63 already holds the tile carry moved across the wave
+          tile_out0 = __builtin_amdgcn_readlane(s0, 63);
+          tile_out1 = SUB == 2 ? __builtin_amdgcn_readlane(s1, 63) : 0;
         }
         // exclusive carry into this lane
-        const int32_t e0 = __shfl_up(s0, 1, 64);
-        const int32_t e1 = __shfl_up(s1, 1, 64);
-        int32_t x0, x1;
-        if (SUB != 2) {
-          x0 = lane == 0 ? w0 : max(e0, w0 + lane * prm.decay);
-          x1 = 0;
-        } else {
-          int32_t u0 = w0, u1 = w1;
-#pragma unroll
-          for (int st = 0; st < 6; ++st) {
-            if ((lane >> st) & 1) MpApply(s_mpow[st], u0, u1);
+        int32_t x0 = Dpp<kWaveShr1, 0xf>(w0, s0), x1 = 0;
+        if constexpr (SUB == 2) x1 = Dpp<kWaveShr1, 0xf>(w1, s1);
+        if (!one_wave && lane > 0) {
+          // the wave carry moved across the lanes before this one
+          if constexpr (SUB != 2) {
+            x0 = max(x0, w0 + ldc);
+          } else {
+            x0 = max(x0, max(lpc[0] + w0, lpc[1] + w1));
+            x1 = max(x1, max(lpc[2] + w0, lpc[3] + w1));
           }
-          x0 = lane == 0 ? w0 : max(e0, u0);
-          x1 = lane == 0 ? w1 : max(e1, u1);
         }
 
-        // ---- 3. exact pass: E, Q, H column by column, packed as produced
-        // (the H1 words are re-read here rather than keeping their unpacked
-        // columns alive across the scan)
-#pragma unroll
-        for (int i = 0; i < KW; ++i) asm volatile("" : "+v"(h1[i]));
+        // ---- 3. fix-up: every column's value is the better of its local
+        // value and the carry moved to it; then H.  E / Q leave in 16-byte
+        // chunks as soon as a chunk is complete, H after the row.
         uint32_t hw[KW];
-        {
-          // E and Q leave in 16-byte chunks as soon as a chunk is complete
-          uint32_t ew[4], qw[4];
-          int32_t pe = x0, pq = x1;  // E, Q at the previous column (before column c0: the carry)
-          int32_t hlo = 0, elo = 0, qlo = 0;
+        if constexpr (CPW == 2) {
+          if constexpr (SUB == 0) {
+            uint32_t v = PkAddSat(Splat<ST>(Clamp16(x0)), Pack16(0, g));
+            const uint32_t k2g = Splat<ST>(2 * g);
 #pragma unroll
-          for (int k = 0; k < K; ++k) {
-            const int32_t h1k = Col<ST, KW>(h1, k);
-            int32_t h;
-            int32_t ek = 0, qk = 0;
-            if (SUB == 0) {
-              h = k == 0 ? max(h1k, x0) : max(h1k, pe + g);
-              pe = h;
-            } else {
-              if (k == 0) {
-                ek = x0;
-                qk = x1;
-              } else {
-                const int32_t hp = Col<ST, KW>(h1, k - 1);
-                if (SUB == 2 && k % L == 0) {
-                  const int32_t hf = max(hp, max(pe, pq));
-                  ek = max(hf + g, pe + e);
-                  qk = max(hf + q, pq + cc);
-                } else {
-                  ek = max(hp + g, pe + e);
-                  if (SUB == 2) qk = max(hp + q, pq + cc);
+            for (int i = 0; i < KW; ++i) {
+              hw[i] = PkMax(loc[i], v);
+              v = PkAddSat(v, k2g);
+            }
+          } else if constexpr (SUB == 1) {
+            uint32_t v = PkAddSat(Splat<ST>(Clamp16(x0)), Pack16(0, e));
+            const uint32_t k2e = Splat<ST>(2 * e);
+#pragma unroll
+            for (int j = 0; j < KW / 4; ++j) {
+              uint32_t ew[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const int i = 4 * j + u;
+                uint32_t x = PkMax(loc[i], v);
+                if (i == 0) x = (x & 0xffff0000u) | (v & 0xffffu);  // E at c0 is the carry
+                ew[u] = x;
+                hw[i] = PkMax(h1[i], x);
+                v = PkAddSat(v, k2e);
+              }
+              if (active) BufStore<4>(rrow, off + 2 * mbytes + 16u * j, ew);
+            }
+          } else {
+            const uint32_t EC = Pack16(e, cc);
+            int32_t cE = x0, cQ = x1;
+            uint32_t v = 0u;
+#pragma unroll
+            for (int j = 0; j < KW / 4; ++j) {
+              // columns 8j .. 8j + 7
+#pragma unroll
+              for (int t = 0; t < 8; ++t) {
+                const int k = 8 * j + t;
+                if (k % L == 0) {
+                  if (k > 0) MpApply(prm.mseg, cE, cQ);
+                  v = Pack16(Clamp16(cE), Clamp16(cQ));
                 }
+                loc[k] = k == 0 ? v : PkMax(loc[k], v);
+                v = PkAddSat(v, EC);
               }
-              h = max(h1k, ek);
-              if (SUB == 2) h = max(h, qk);
-              pe = ek;
-              pq = qk;
-            }
-            const int32_t hs = type == 0 ? max(h, 0) : h;
-            const int wi = k / CPW;  // word of column k
-            bool word_done = true;
-            if constexpr (CPW == 2) {
-              if (k & 1) {
-                hw[wi] = (static_cast<uint32_t>(hlo) & 0xffffu) | (static_cast<uint32_t>(hs) << 16);
-                if (SUB >= 1) ew[wi % 4] = (static_cast<uint32_t>(elo) & 0xffffu) | (static_cast<uint32_t>(ek) << 16);
-                if (SUB == 2) qw[wi % 4] = (static_cast<uint32_t>(qlo) & 0xffffu) | (static_cast<uint32_t>(qk) << 16);
-              } else {
-                hlo = hs;
-                elo = ek;
-                qlo = qk;
-                word_done = false;
+              uint32_t ew[4], qw[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const int i = 4 * j + u;
+                ew[u] = __builtin_amdgcn_perm(loc[2 * i + 1], loc[2 * i], 0x05040100u);
+                qw[u] = __builtin_amdgcn_perm(loc[2 * i + 1], loc[2 * i], 0x07060302u);
+                hw[i] = PkMax(h1[i], PkMax(ew[u], qw[u]));
               }
-            } else {
-              hw[wi] = static_cast<uint32_t>(hs);
-              if (SUB >= 1) ew[wi % 4] = static_cast<uint32_t>(ek);
-              if (SUB == 2) qw[wi % 4] = static_cast<uint32_t>(qk);
-            }
-            if (SUB >= 1 && word_done && wi % 4 == 3 && active) {
-              const uint32_t at = off + 16u * (wi / 4);
-              BufStore<4>(rrow, at + 2 * mbytes, ew);
-              if (SUB == 2) BufStore<4>(rrow, at + 4 * mbytes, qw);
+              if (active) {
+                BufStore<4>(rrow, off + 2 * mbytes + 16u * j, ew);
+                BufStore<4>(rrow, off + 4 * mbytes + 16u * j, qw);
+              }
             }
           }
-          if (active) BufStore<KW>(rrow, off, hw);
+          if (type == 0) {
+#pragma unroll
+            for (int i = 0; i < KW; ++i) hw[i] = PkMax(hw[i], 0u);
+          }
+        } else {
+          if constexpr (SUB == 0) {
+            int32_t v = x0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              hw[k] = static_cast<uint32_t>(max(static_cast<int32_t>(loc[k]), v));
+              v += g;
+            }
+          } else if constexpr (SUB == 1) {
+            int32_t v = x0;
+#pragma unroll
+            for (int j = 0; j < K / 4; ++j) {
+              uint32_t ew[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const int k = 4 * j + u;
+                const int32_t ek = k == 0 ? v : max(static_cast<int32_t>(loc[k]), v);
+                ew[u] = static_cast<uint32_t>(ek);
+                hw[k] = static_cast<uint32_t>(max(static_cast<int32_t>(h1[k]), ek));
+                v += e;
+              }
+              if (active) BufStore<4>(rrow, off + 2 * mbytes + 16u * j, ew);
+            }
+          } else {
+            int32_t cE = x0, cQ = x1, vE = 0, vQ = 0;
+#pragma unroll
+            for (int j = 0; j < K / 4; ++j) {
+              uint32_t ew[4], qw[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const int k = 4 * j + u;
+                if (k % L == 0) {
+                  if (k > 0) MpApply(prm.mseg, cE, cQ);
+                  vE = cE;
+                  vQ = cQ;
+                }
+                const int32_t ek = k == 0 ? vE : max(static_cast<int32_t>(loc[k]), vE);
+                const int32_t qk = k == 0 ? vQ : max(static_cast<int32_t>(qloc[k]), vQ);
+                ew[u] = static_cast<uint32_t>(ek);
+                qw[u] = static_cast<uint32_t>(qk);
+                hw[k] = static_cast<uint32_t>(max(static_cast<int32_t>(h1[k]), max(ek, qk)));
+                vE += e;
+                vQ += cc;
+              }
+              if (active) {
+                BufStore<4>(rrow, off + 2 * mbytes + 16u * j, ew);
+                BufStore<4>(rrow, off + 4 * mbytes + 16u * j, qw);
+              }
+            }
+          }
+          if (type == 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) hw[k] = static_cast<uint32_t>(max(static_cast<int32_t>(hw[k]), 0));
+          }
         }
+        if (active) BufStore<KW>(rrow, off, hw);
         if (single) {
 #pragma unroll
           for (int i = 0; i < KW; ++i) cH[i] = hw[i];
