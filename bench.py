@@ -116,12 +116,130 @@ def cpu_baseline(windows, params, strand, budget_s, threads):
             "consensus": results}
 
 
+def pct(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = min(len(xs) - 1, max(0, int(round(p / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def cpu_serving(jobs, arrivals, threads, budget_s):
+    """The same arrival trace into a pool of host threads running a CPU
+    engine (the reference's AVX2 build where present, else the oracle), one
+    job per thread at a time; stops admitting after budget_s.  Returns
+    latency percentiles and throughput over the jobs that ran."""
+    import threading
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libspoa_ref.so")
+    port_so = os.path.join(ROOT, "oracle", "liboracle.so")
+    if os.path.exists(ref_so):
+        kind, fn = "reference", ctypes.CDLL(ref_so).ref_run_window
+    elif os.path.exists(port_so):
+        kind, fn = "port", ctypes.CDLL(port_so).oracle_run_window
+    else:
+        return None
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int] * 7 + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int,
+                                        ctypes.c_char_p, ctypes.c_uint32]
+    lat, lock = [], threading.Lock()
+    t0 = time.perf_counter()
+
+    def run(i):
+        reads, params = jobs[i]
+        t, m, n, g, e, q, c = tuple(params) + (None,) * (7 - len(params))
+        e = g if e is None else e
+        if q is None:
+            q, c = g, e
+        data = "".join(reads).encode()
+        offs = [0]
+        for r in reads:
+            offs.append(offs[-1] + len(r))
+        arr = (ctypes.c_uint64 * len(offs))(*offs)
+        out = ctypes.create_string_buffer(1 << 16)
+        fn(t, m, n, g, e, q, c, len(reads), arr, data, 0, out, 1 << 16)
+        with lock:
+            lat.append(1000.0 * (time.perf_counter() - t0 - arrivals[i]))
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
+        futs = []
+        for i in range(len(jobs)):
+            dt = arrivals[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            if time.perf_counter() - t0 > budget_s:
+                break
+            futs.append(ex.submit(run, i))
+        for f in futs:
+            f.result()
+    el = time.perf_counter() - t0
+    return {"kind": kind, "cores": threads, "jobs": len(lat), "windows_per_s": len(lat) / el,
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99)}
+
+
+def serving_bench(args, threads):
+    """Config 5: Poisson arrivals of serving-mix jobs at several offered
+    loads; throughput and p50 / p99 latency (submit to consensus + MSA)."""
+    cfg = spoa_amd.CONFIGS[5]
+    rates = [float(x) for x in args.rates.split(",")]
+    srv = spoa_amd.Server(threads=threads)
+    sweep = []
+    try:
+        # warm-up: a few jobs through both parameter sets
+        wj, wa = spoa_amd.serving_jobs(args.seed + 1, 8, 1000.0, template=(200, 2000), reads=(5, 20))
+        spoa_amd.serve_trace(wj, wa, server=srv)
+        for k, rate in enumerate(rates):
+            n = max(8, int(rate * args.serve_seconds))
+            jobs, arrivals = spoa_amd.serving_jobs(args.seed + 10 + k, n, rate)
+            before = srv.stats()
+            t0 = time.perf_counter()
+            res, wall, st = spoa_amd.serve_trace(jobs, arrivals, server=srv)
+            lat = [r.latency_ms for r in res]
+            ok = sum(1 for r in res if r.status == 0)
+            kern = st["kernel_ms"] - before["kernel_ms"]
+            byts = st["dp_bytes"] - before["dp_bytes"]
+            launches = st["launches"] - before["launches"]
+            sweep.append({"offered_per_s": rate, "jobs": n, "completed": ok, "wall_s": wall,
+                          "windows_per_s": ok / wall, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                          "gcups_dp": (st["cells"] - before["cells"]) / (kern / 1e3) / 1e9 if kern else 0.0,
+                          "kernel_ms": kern, "launches": launches,
+                          "achieved_gbs": byts / (kern / 1e3) / 1e9 if kern else 0.0,
+                          "deferred": st["deferred"] - before["deferred"],
+                          "max_active": st["max_active_seen"]})
+            print("serving rate %.1f/s: %d jobs, %.1f windows/s, p50 %.0f ms, p99 %.0f ms" % (
+                rate, n, ok / wall, pct(lat, 50), pct(lat, 99)), file=sys.stderr, flush=True)
+    finally:
+        srv.close()
+    top = sweep[-1]
+    cpu = None
+    if not args.no_cpu:
+        jobs, arrivals = spoa_amd.serving_jobs(args.seed + 10, max(8, int(rates[0] * args.serve_seconds)), rates[0])
+        cpu = cpu_serving(jobs, arrivals, threads, args.cpu_budget)
+    achieved = top["achieved_gbs"]
+    out = {
+        "metric": "windows/s (serving: continuous batching, consensus + MSA per job)",
+        "value": top["windows_per_s"], "unit": "windows/s", "n_gpus": 1, "steps": len(rates), "warmup": 1,
+        "ms_per_step": 1000.0 * top["wall_s"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int16/int32", "data": "synthetic (seeded serving mix, Poisson arrivals)",
+        "config": {"workload": cfg["name"], "rates_per_s": rates, "seconds_per_rate": args.serve_seconds,
+                   "seed": args.seed, "host_threads": threads},
+        "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "PoaAlignKernel (serving rounds)",
+                     "launches": top["launches"]},
+        "sweep": sweep,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4])
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
+    ap.add_argument("--rates", default="10,25,50", help="config 5: offered loads, jobs/s")
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="config 5: arrival span per load")
     ap.add_argument("--windows", type=int, default=None, help="windows per step (default per config)")
     ap.add_argument("--seed", type=int, default=20240601)
     ap.add_argument("--threads", type=int, default=0, help="host threads (0: all, at most 16)")
@@ -145,9 +263,12 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    threads = args.threads or min(16, os.cpu_count() or 1)
+    if args.config == 5:
+        serving_bench(args, threads)
+        return
     cfg = spoa_amd.CONFIGS[args.config]
     n_windows = args.windows or {1: 1024, 2: 2048, 3: 4096, 4: 64}[args.config]
-    threads = args.threads or min(16, os.cpu_count() or 1)
     params = cfg["params"]
     strand = cfg.get("strand", False)
     windows = spoa_amd.config_windows(args.config, args.seed, n_windows)

@@ -114,6 +114,45 @@ int64_t spoa_batch_consensus_all(void* result, char* out, uint64_t cap, uint64_t
 int spoa_batch_msa(void* result, uint32_t window, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols);
 int spoa_batch_reversed(void* result, uint32_t window, uint8_t* out, uint32_t cap);
 
+/* ----------------------------------------------------------- serving -- */
+/* A serving loop on one GPU (continuous batching of consensus jobs, each a
+ * window of reads with its own scoring parameters; DP memory admitted per
+ * round).  No counterpart in the reference. */
+typedef struct spoa_serve_opts {
+  int32_t threads;          /* host threads, 0 = all */
+  int32_t device;
+  uint32_t max_active;      /* job slots */
+  uint32_t max_param_sets;  /* scoring parameter sets served at once */
+  double memory_fraction;   /* of the free device memory, for DP matrices */
+} spoa_serve_opts;
+
+typedef struct spoa_serve_stats {
+  uint64_t submitted, completed, failed, rounds, launches, align_calls, deferred, cells, dp_bytes;
+  double kernel_ms;
+  uint32_t max_active_seen;
+} spoa_serve_stats;
+
+void* spoa_server_new(const spoa_serve_opts* opts);
+/* finishes every submitted job, then stops */
+void spoa_server_free(void* server);
+/* One job: Create(type, m, n, g, e, q, c) parameters, reads
+ * seqs[read_off[k], read_off[k+1]) (quals likewise or NULL), msa != 0 to
+ * build the MSA too.  Returns the job id, 0 on error (spoa_last_error). */
+uint64_t spoa_server_submit(void* server, int type, int m, int n, int g, int e, int q, int c, uint32_t n_reads,
+                            const uint64_t* read_off, const char* seqs, const char* quals, int msa);
+/* Waits for a job (timeout_s < 0: no limit); returns its result, NULL on
+ * timeout.  Each result is returned once. */
+void* spoa_server_wait(void* server, uint64_t id, double timeout_s);
+int spoa_server_stats(void* server, spoa_serve_stats* out);
+double spoa_server_now_ms(void* server);
+/* result: status 0 ok / -1 failed; times in ms since the server started */
+int spoa_serve_result_info(void* result, int* status, double* submit_ms, double* admit_ms, double* done_ms,
+                           uint32_t* rounds);
+const char* spoa_serve_result_error(void* result);
+int spoa_serve_result_consensus(void* result, char* out, uint32_t cap);
+int spoa_serve_result_msa(void* result, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols);
+void spoa_serve_result_free(void* result);
+
 /* ------------------------------------------------------------- synth -- */
 typedef struct spoa_synth_spec {
   uint32_t template_min, template_max; /* template length range */
@@ -121,7 +160,8 @@ typedef struct spoa_synth_spec {
   uint32_t read_len_min, read_len_max; /* 0, 0: every read covers the template */
   double err_min, err_max;             /* per-window error rate range */
   uint32_t revcomp_odd;                /* odd-numbered reads reverse-complemented */
-  uint32_t log_uniform_len;            /* read length log-uniform in its range */
+  uint32_t log_uniform_len;            /* read length log-uniform in its range (the template's
+                                          length when reads span the template) */
 } spoa_synth_spec;
 
 void* spoa_synth_new(const spoa_synth_spec* spec, uint64_t seed, uint32_t n_windows);

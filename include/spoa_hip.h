@@ -113,6 +113,10 @@ int spoa_hip_align_batch(spoa_hip_engine* engine, spoa_hip_job* jobs, uint32_t n
  * engine finishes a pending batch first. */
 int spoa_hip_align_batch_submit(spoa_hip_engine* engine, spoa_hip_job* jobs, uint32_t n_jobs);
 int spoa_hip_align_batch_wait(spoa_hip_engine* engine);
+/* No counterpart in the reference: 1 when a submitted batch has finished on
+ * the device (or none is pending), so that wait will not block; 0 while it
+ * runs; negative (a status) on a bad argument. */
+int spoa_hip_align_batch_ready(spoa_hip_engine* engine);
 
 /* Measurements of the last align / align_batch call: device time of the
  * alignment kernels from HIP events on the engine's stream, copies, launch

@@ -65,6 +65,20 @@ class BatchStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServeOpts(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_int32), ("device", ctypes.c_int32), ("max_active", ctypes.c_uint32),
+                ("max_param_sets", ctypes.c_uint32), ("memory_fraction", ctypes.c_double)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "completed", "failed", "rounds", "launches",
+                                                 "align_calls", "deferred", "cells", "dp_bytes")] + \
+               [("kernel_ms", ctypes.c_double), ("max_active_seen", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SynthSpec(ctypes.Structure):
     _fields_ = [("template_min", ctypes.c_uint32), ("template_max", ctypes.c_uint32),
                 ("reads_min", ctypes.c_uint32), ("reads_max", ctypes.c_uint32),
@@ -134,6 +148,24 @@ def lib():
     L.spoa_synth_sizes.argtypes = [ctypes.c_void_p, _u32p, _u64p, _u64p, _u64p]
     L.spoa_synth_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
                                   ctypes.c_void_p, ctypes.c_char_p]
+    L.spoa_server_new.restype = ctypes.c_void_p
+    L.spoa_server_new.argtypes = [ctypes.POINTER(ServeOpts)]
+    L.spoa_server_free.argtypes = [ctypes.c_void_p]
+    L.spoa_server_submit.restype = ctypes.c_uint64
+    L.spoa_server_submit.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 7 + [
+        ctypes.c_uint32, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+    L.spoa_server_wait.restype = ctypes.c_void_p
+    L.spoa_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double]
+    L.spoa_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServeStats)]
+    L.spoa_server_now_ms.restype = ctypes.c_double
+    L.spoa_server_now_ms.argtypes = [ctypes.c_void_p]
+    L.spoa_serve_result_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)] + \
+        [ctypes.POINTER(ctypes.c_double)] * 3 + [_u32p]
+    L.spoa_serve_result_error.restype = ctypes.c_char_p
+    L.spoa_serve_result_error.argtypes = [ctypes.c_void_p]
+    L.spoa_serve_result_consensus.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32]
+    L.spoa_serve_result_msa.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64, _u32p, _u32p]
+    L.spoa_serve_result_free.argtypes = [ctypes.c_void_p]
     # C ABI of the kernels (include/spoa_hip.h), for tuning and budgets
     L.spoa_hip_last_error.restype = ctypes.c_char_p
     L.spoa_hip_set_config.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
@@ -480,6 +512,137 @@ def _collect(h, windows, msa, strand):
     return WindowResults(h, windows, msa, strand)
 
 
+class ServeResult:
+    def __init__(self, status, error, consensus, msa, submit_ms, admit_ms, done_ms, rounds):
+        self.status = status
+        self.error = error
+        self.consensus = consensus
+        self.msa = msa
+        self.submit_ms = submit_ms
+        self.admit_ms = admit_ms
+        self.done_ms = done_ms
+        self.rounds = rounds
+
+    @property
+    def latency_ms(self):
+        return self.done_ms - self.submit_ms
+
+
+class Server:
+    """The serving loop on one GPU: consensus jobs (a window of reads and its
+    scoring parameters) are admitted as slots and device memory free up and
+    advance one read per round in continuous batches."""
+
+    def __init__(self, threads=0, device=0, max_active=4096, max_param_sets=2, memory_fraction=0.6):
+        o = ServeOpts(threads, device, max_active, max_param_sets, memory_fraction)
+        self._h = lib().spoa_server_new(ctypes.byref(o))
+        if not self._h:
+            raise ValueError(_error())
+
+    def close(self):
+        """Finishes every submitted job, then stops the server."""
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.spoa_server_free(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def submit(self, reads, params, qualities=None, msa=True):
+        t, m, n, g, e, q, c = tuple(params) + (None,) * (7 - len(params))
+        e = g if e is None else e
+        if q is None:
+            q, c = g, e
+        enc = [_b(r) for r in reads]
+        off = np.zeros(len(enc) + 1, dtype=np.uint64)
+        off[1:] = np.cumsum([len(x) for x in enc]) if enc else []
+        quals = b"".join(_b(x) for x in qualities) if qualities is not None else None
+        jid = lib().spoa_server_submit(self._h, t, m, n, g, e, q, c, len(enc), off.ctypes.data, b"".join(enc),
+                                       quals, int(msa))
+        if jid == 0:
+            raise ValueError(_error())
+        return jid
+
+    def now_ms(self):
+        return lib().spoa_server_now_ms(self._h)
+
+    def wait(self, jid, timeout=-1.0):
+        """The job's ServeResult, or None on timeout."""
+        r = lib().spoa_server_wait(self._h, jid, timeout)
+        if not r:
+            return None
+        try:
+            st = ctypes.c_int(0)
+            t = [ctypes.c_double(0) for _ in range(3)]
+            rounds = ctypes.c_uint32(0)
+            lib().spoa_serve_result_info(r, ctypes.byref(st), ctypes.byref(t[0]), ctypes.byref(t[1]),
+                                         ctypes.byref(t[2]), ctypes.byref(rounds))
+            err = lib().spoa_serve_result_error(r).decode()
+            cap = 1 << 16
+            while True:
+                buf = ctypes.create_string_buffer(cap)
+                k = lib().spoa_serve_result_consensus(r, buf, cap)
+                if k == -2:
+                    cap *= 4
+                    continue
+                break
+            cons = buf.raw[:k].decode()
+            rows, cols = ctypes.c_uint32(0), ctypes.c_uint32(0)
+            cap = 1 << 16
+            while True:
+                mb = ctypes.create_string_buffer(cap)
+                rc = lib().spoa_serve_result_msa(r, mb, cap, ctypes.byref(rows), ctypes.byref(cols))
+                if rc == -2:
+                    cap = rows.value * cols.value + 1
+                    continue
+                break
+            raw = mb.raw
+            msa = [raw[i * cols.value:(i + 1) * cols.value].decode() for i in range(rows.value)]
+            return ServeResult(st.value, err, cons, msa, t[0].value, t[1].value, t[2].value, rounds.value)
+        finally:
+            lib().spoa_serve_result_free(r)
+
+    def stats(self):
+        st = ServeStats()
+        lib().spoa_server_stats(self._h, ctypes.byref(st))
+        return st.as_dict()
+
+
+def serve_trace(jobs, arrivals_s, server=None, msa=True, **server_args):
+    """Replays an arrival trace: job i (reads, params) is submitted at
+    arrivals_s[i] seconds after the start; returns (results in job order,
+    wall seconds from the first arrival to the last completion, stats)."""
+    import threading
+    import time
+    own = server is None
+    srv = Server(**server_args) if own else server
+    ids = [None] * len(jobs)
+    err = []
+
+    def feeder():
+        t0 = time.perf_counter()
+        try:
+            for i, (reads, params) in enumerate(jobs):
+                dt = arrivals_s[i] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                ids[i] = srv.submit(reads, params, msa=msa)
+        except BaseException as ex:  # surfaced below
+            err.append(ex)
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=feeder)
+    th.start()
+    th.join()
+    if err:
+        raise err[0]
+    results = [srv.wait(j) for j in ids]
+    wall = time.perf_counter() - t0
+    stats = srv.stats()
+    if own:
+        srv.close()
+    return results, wall, stats
+
+
 def synth_windows(seed, n_windows, template=(500, 500), reads=(20, 20), read_len=(0, 0),
                   err=(0.1, 0.1), revcomp_odd=False, log_uniform_len=False):
     """Seeded synthetic windows: uniform ACGT templates, reads as template
@@ -517,6 +680,12 @@ CONFIGS = {
     4: dict(name="cfg4_nw_convex_5-10kb_256windows_strand", n_windows=256, template=(5000, 10000),
             reads=(8, 32), read_len=(0, 0), err=(0.10, 0.10), revcomp_odd=True, strand=True,
             params=(NW, 5, -4, -8, -6, -10, -4)),
+    # serving: jobs of both the cfg-2 and the cfg-3 scoring, reads spanning a
+    # template of log-uniform length; job i takes params_mix[i % 2]
+    5: dict(name="cfg5_serving_mix_200bp-10kb", n_windows=512, template=(200, 10000), reads=(5, 100),
+            read_len=(0, 0), err=(0.08, 0.15), log_uniform_len=True,
+            params=(SW, 5, -4, -8, -6, -10, -4),
+            params_mix=((SW, 5, -4, -8, -6, -10, -4), (OV, 5, -4, -8, -6))),
 }
 
 
@@ -524,4 +693,21 @@ def config_windows(cfg, seed, n_windows=None):
     c = CONFIGS[cfg]
     return synth_windows(seed, c["n_windows"] if n_windows is None else n_windows,
                          template=c["template"], reads=c["reads"], read_len=c["read_len"],
-                         err=c["err"], revcomp_odd=c.get("revcomp_odd", False))
+                         err=c["err"], revcomp_odd=c.get("revcomp_odd", False),
+                         log_uniform_len=c.get("log_uniform_len", False))
+
+
+def serving_jobs(seed, n_jobs, rate_per_s, cfg=5, template=None, reads=None):
+    """Seeded serving workload: n_jobs (reads, params) windows of the serving
+    mix and their Poisson arrival times (seconds) at rate_per_s."""
+    import random
+    c = CONFIGS[cfg]
+    ws = synth_windows(seed, n_jobs, template=template or c["template"], reads=reads or c["reads"],
+                       read_len=c["read_len"], err=c["err"], log_uniform_len=c.get("log_uniform_len", False))
+    rng = random.Random(seed)
+    t, arrivals = 0.0, []
+    for _ in range(n_jobs):
+        t += rng.expovariate(rate_per_s)
+        arrivals.append(t)
+    jobs = [(ws.reads(i), c["params_mix"][i % len(c["params_mix"])]) for i in range(n_jobs)]
+    return jobs, arrivals

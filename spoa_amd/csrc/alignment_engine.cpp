@@ -41,6 +41,16 @@ std::unique_ptr<AlignmentEngine> AlignmentEngine::Create(AlignmentType type, std
 std::unique_ptr<AlignmentEngine> AlignmentEngine::Create(AlignmentType type, std::int8_t m, std::int8_t n,
                                                          std::int8_t g, std::int8_t e, std::int8_t q,
                                                          std::int8_t c) {
+  const AlignmentSubtype subtype = CheckCreateParams(type, &g, &e, &q, &c);
+  return std::unique_ptr<AlignmentEngine>(new HipAlignmentEngine(type, subtype, m, n, g, e, q, c));
+}
+
+AlignmentSubtype CheckCreateParams(AlignmentType type, std::int8_t* g_, std::int8_t* e_, std::int8_t* q_,
+                                   std::int8_t* c_) {
+  std::int8_t& g = *g_;
+  std::int8_t& e = *e_;
+  std::int8_t& q = *q_;
+  std::int8_t& c = *c_;
   if (type != AlignmentType::kSW && type != AlignmentType::kNW && type != AlignmentType::kOV) {
     throw std::invalid_argument("[spoa::AlignmentEngine::Create] error: invalid alignment type!");
   }
@@ -63,7 +73,7 @@ std::unique_ptr<AlignmentEngine> AlignmentEngine::Create(AlignmentType type, std
     q = g;
     c = e;
   }
-  return std::unique_ptr<AlignmentEngine>(new HipAlignmentEngine(type, subtype, m, n, g, e, q, c));
+  return subtype;
 }
 
 AlignmentEngine::AlignmentEngine(AlignmentType type, AlignmentSubtype subtype, std::int8_t m, std::int8_t n,

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "serve.hpp"
 #include "poa_graph.hpp"
 #include "spoa/spoa.hpp"
 #include "spoa_amd_host.hpp"
@@ -365,6 +366,107 @@ int spoa_batch_reversed(void* h, uint32_t w, uint8_t* out, uint32_t cap) {
   std::memcpy(out, v.data(), v.size());
   return static_cast<int>(v.size());
 }
+
+// ----------------------------------------------------------- serving --
+
+void* spoa_server_new(const spoa_serve_opts* o) {
+  try {
+    spoa_amd::ServeOptions so;
+    if (o) {
+      so.threads = o->threads > 0 ? static_cast<unsigned>(o->threads) : 0u;
+      so.device = o->device;
+      if (o->max_active) so.max_active = o->max_active;
+      if (o->max_param_sets) so.max_param_sets = o->max_param_sets;
+      if (o->memory_fraction > 0) so.memory_fraction = o->memory_fraction;
+    }
+    return new spoa_amd::Server(so);
+  } catch (const std::exception& ex) {
+    Error(ex);
+    return nullptr;
+  }
+}
+
+void spoa_server_free(void* srv) { delete static_cast<spoa_amd::Server*>(srv); }
+
+uint64_t spoa_server_submit(void* srv, int type, int m, int n, int g, int e, int q, int c, uint32_t n_reads,
+                            const uint64_t* read_off, const char* seqs, const char* quals, int msa) {
+  try {
+    spoa_amd::ServeJob job;
+    job.params.type = static_cast<spoa::AlignmentType>(type);
+    job.params.m = static_cast<std::int8_t>(m);
+    job.params.n = static_cast<std::int8_t>(n);
+    job.params.g = static_cast<std::int8_t>(g);
+    job.params.e = static_cast<std::int8_t>(e);
+    job.params.q = static_cast<std::int8_t>(q);
+    job.params.c = static_cast<std::int8_t>(c);
+    job.msa = msa != 0;
+    job.reads.reserve(n_reads);
+    for (uint32_t k = 0; k < n_reads; ++k) {
+      job.reads.emplace_back(seqs + read_off[k], seqs + read_off[k + 1]);
+      if (quals) job.quals.emplace_back(quals + read_off[k], quals + read_off[k + 1]);
+    }
+    return static_cast<spoa_amd::Server*>(srv)->Submit(std::move(job));
+  } catch (const std::exception& ex) {
+    Error(ex);
+    return 0;
+  }
+}
+
+void* spoa_server_wait(void* srv, uint64_t id, double timeout_s) {
+  std::unique_ptr<spoa_amd::ServeResult> r(new spoa_amd::ServeResult());
+  if (!static_cast<spoa_amd::Server*>(srv)->Wait(id, r.get(), timeout_s)) return nullptr;
+  return r.release();
+}
+
+int spoa_server_stats(void* srv, spoa_serve_stats* out) {
+  const spoa_amd::ServeStats s = static_cast<spoa_amd::Server*>(srv)->Stats();
+  out->submitted = s.submitted;
+  out->completed = s.completed;
+  out->failed = s.failed;
+  out->rounds = s.rounds;
+  out->launches = s.launches;
+  out->align_calls = s.align_calls;
+  out->deferred = s.deferred;
+  out->cells = s.cells;
+  out->dp_bytes = s.dp_bytes;
+  out->kernel_ms = s.kernel_ms;
+  out->max_active_seen = s.max_active_seen;
+  return 0;
+}
+
+double spoa_server_now_ms(void* srv) { return static_cast<spoa_amd::Server*>(srv)->NowMs(); }
+
+int spoa_serve_result_info(void* res, int* status, double* submit_ms, double* admit_ms, double* done_ms,
+                           uint32_t* rounds) {
+  const auto& r = *static_cast<spoa_amd::ServeResult*>(res);
+  if (status) *status = r.status;
+  if (submit_ms) *submit_ms = r.submit_ms;
+  if (admit_ms) *admit_ms = r.admit_ms;
+  if (done_ms) *done_ms = r.done_ms;
+  if (rounds) *rounds = r.rounds;
+  return 0;
+}
+
+const char* spoa_serve_result_error(void* res) { return static_cast<spoa_amd::ServeResult*>(res)->error.c_str(); }
+
+int spoa_serve_result_consensus(void* res, char* out, uint32_t cap) {
+  const std::string& s = static_cast<spoa_amd::ServeResult*>(res)->consensus;
+  if (s.size() > cap) return -2;
+  std::memcpy(out, s.data(), s.size());
+  return static_cast<int>(s.size());
+}
+
+int spoa_serve_result_msa(void* res, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols) {
+  const auto& m = static_cast<spoa_amd::ServeResult*>(res)->msa;
+  *rows = static_cast<uint32_t>(m.size());
+  *cols = m.empty() ? 0 : static_cast<uint32_t>(m[0].size());
+  const uint64_t need = static_cast<uint64_t>(*rows) * *cols;
+  if (need > cap) return -2;
+  for (std::size_t k = 0; k < m.size(); ++k) std::memcpy(out + k * (*cols), m[k].data(), *cols);
+  return 0;
+}
+
+void spoa_serve_result_free(void* res) { delete static_cast<spoa_amd::ServeResult*>(res); }
 
 // ------------------------------------------------------------- synth --
 

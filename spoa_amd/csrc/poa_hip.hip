@@ -333,8 +333,16 @@ Cfg ChooseCfg(const spoa_hip_engine* E, uint32_t len, int width_bits, bool batch
   return DefaultCfg(E, W, width_bits, batch);
 }
 
-uint64_t MatBytes(const spoa_hip_params& p, uint32_t V, uint32_t stride, int width) {
-  return RoundUp(static_cast<uint64_t>(V + 1) * Matrices(p.subtype) * stride * (width / 8), kAlign);
+// Device bytes of one job: its DP matrices, plus -- when a row spans
+// several wave segments (waves x tiles) -- the H value left of every
+// segment for every row (int32), which the first lane of a segment keeps
+// for the diagonal moves of later rows.
+uint64_t JobBytes(const spoa_hip_params& p, uint32_t V, uint32_t stride, int width, const Cfg& cfg) {
+  uint64_t b = static_cast<uint64_t>(V + 1) * Matrices(p.subtype) * stride * (width / 8);
+  const uint64_t tile = 64ull * static_cast<uint64_t>(cfg.k) * static_cast<uint64_t>(cfg.waves);
+  const uint64_t nseg = (stride + tile - 1) / tile * static_cast<uint64_t>(cfg.waves);
+  if (nseg > 1) b += nseg * (V + 1) * 4ull;
+  return RoundUp(b, kAlign);
 }
 
 // Packs one job's graph block and read into the staging buffer.
@@ -739,7 +747,7 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     const int L = Lanes(width);
     pl.W = (j.len + L - 1) / L * L;
     pl.stride = static_cast<uint32_t>(RoundUp(pl.W, static_cast<uint64_t>(pl.cfg.k)));
-    pl.mat_bytes = MatBytes(P, V, pl.stride, width);
+    pl.mat_bytes = JobBytes(P, V, pl.stride, width, pl.cfg);
     const uint32_t E_in = j.graph.pred_off[V] - j.graph.pred_off[0];
     pl.graph_words = RoundUp(9ull * V + 4 + E_in, 4);
     pl.seq_bytes = RoundUp(pl.stride + 16, 16);
@@ -877,8 +885,8 @@ int spoa_hip_prealloc(spoa_hip_engine* E, uint32_t max_len, uint32_t alphabet_si
   const int L = Lanes(width);
   const uint64_t W = (static_cast<uint64_t>(max_len) + L - 1) / L * L;
   const uint64_t stride = RoundUp(W, static_cast<uint64_t>(cfg.k));
-  const uint64_t bytes =
-      RoundUp(static_cast<uint64_t>(V + 1) * Matrices(E->prm.subtype) * stride * (width / 8), kAlign);
+  if (V >= static_cast<int64_t>(std::numeric_limits<uint32_t>::max())) return Fail(SPOA_HIP_NO_MEMORY, kMsgPreNoMemory);
+  const uint64_t bytes = JobBytes(E->prm, static_cast<uint32_t>(V), static_cast<uint32_t>(stride), width, cfg);
   if (bytes > E->budget) return Fail(SPOA_HIP_NO_MEMORY, kMsgPreNoMemory);
   uint64_t largest = 0;
   for (const auto& sg : E->arena) largest = std::max(largest, sg.size);
@@ -933,6 +941,15 @@ int spoa_hip_align_batch_wait(spoa_hip_engine* E) {
   if (rc != SPOA_HIP_OK) return rc;
   if (E->pending.first != SPOA_HIP_OK) g_last_error = before;
   return E->pending.first;
+}
+
+int spoa_hip_align_batch_ready(spoa_hip_engine* E) {
+  if (!E) return -Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null engine");
+  if (!E->pending.active) return 1;
+  const hipError_t q = hipStreamQuery(E->stream);
+  if (q == hipErrorNotReady) return 0;
+  (void)hipGetLastError();
+  return 1;  // finished, or failed: wait reports which
 }
 
 int spoa_hip_last_stats(spoa_hip_engine* E, spoa_hip_stats* out) {
@@ -1008,7 +1025,7 @@ uint64_t spoa_hip_job_footprint(spoa_hip_engine* E, uint32_t V, uint32_t len, in
   const int L = Lanes(width);
   const uint32_t W = (len + L - 1) / L * L;
   const uint32_t stride = static_cast<uint32_t>(RoundUp(W, static_cast<uint64_t>(cfg.k)));
-  return MatBytes(E->prm, V, stride, width);
+  return JobBytes(E->prm, V, stride, width, cfg);
 }
 
 }  // extern "C"

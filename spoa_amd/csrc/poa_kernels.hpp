@@ -158,6 +158,10 @@ __device__ __forceinline__ int32_t Dpp(int32_t old, int32_t v) {
   return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
 }
 
+// Barrier for LDS traffic only: waits for this wave's LDS operations, not
+// for its global loads and stores (which __syncthreads would also drain).
+__device__ __forceinline__ void LdsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 __device__ __forceinline__ uint32_t Uniform(uint32_t v) {
   return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
 }
@@ -547,6 +551,7 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
   constexpr int32_t kMinI = -2147483647 - 1;
 
   __shared__ int32_t s_tot[2][kMaxWaves][2];
+  __shared__ int32_t s_bnd[kMaxWaves];
   __shared__ int32_t s_red[kMaxWaves][3];
   __shared__ uint32_t s_job;
   extern __shared__ uint32_t s_prof[];
@@ -606,7 +611,15 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
     const uint32_t tile_w = static_cast<uint32_t>(nthreads) * K;
     const uint32_t ntiles = (stride + tile_w - 1) / tile_w;
     const bool single = ntiles == 1;  // a row is one tile: the previous row stays in registers
-    const bool sync = nwaves > 1 || ntiles > 1;
+    // A row is nseg = ntiles * nwaves wave segments.  The first lane of
+    // segment sg >= 1 needs, for the diagonal, the H value left of it in
+    // every predecessor row: it keeps them in bnd[sg][row] (its own stores
+    // and loads), received at the end of each row through LDS or, within
+    // one wave, from lane 63.  No wave reads another wave's global stores
+    // while the rows run, so the row barriers only order LDS.
+    const uint32_t nseg = ntiles * static_cast<uint32_t>(nwaves);
+    int32_t* __restrict__ bnd =
+        reinterpret_cast<int32_t*>(reinterpret_cast<uint8_t*>(mat) + static_cast<uint64_t>(V + 1) * rbytes);
     const uint32_t nslots = single ? J.nslots : 0u;
     auto row_ptr = [&](uint32_t r) -> ST* { return mat + static_cast<uint64_t>(r) * stride * NM; };
 
@@ -674,6 +687,11 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
     uint32_t pfH[KW], pfF[KW], pfO[KW];
     bool pf_valid = false;
     uint32_t pf_row = 0;
+    // one-tile rows: H of the previous row left of this wave's segment
+    int32_t bprev = 0;
+    if (single && wave > 0) {
+      bprev = static_cast<int32_t>(static_cast<ST>(row0_h(wave * 64 * K - 1)));
+    }
 
     for (uint32_t r = 1; r <= V; ++r) {
       // row record: base | sink << 8 | profile slot << 9 | predecessor count
@@ -740,8 +758,12 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
           if (lane == 0) {
             if (c0 == 0) {
               hm1 = p + 1 == r ? fch_prev : gv.fcH[p];
+            } else if (p == 0) {
+              hm1 = static_cast<int32_t>(static_cast<ST>(row0_h(static_cast<int32_t>(c0) - 1)));
+            } else if (single && p + 1 == r) {
+              hm1 = bprev;
             } else {
-              hm1 = BufLoadCol<ST>(rp, active ? off - S : rbytes);
+              hm1 = bnd[static_cast<uint64_t>(t * static_cast<uint32_t>(nwaves) + wave) * (V + 1) + p];
             }
           }
           // diagonal (:838-850): H_p[c - 1] + profile
@@ -1002,7 +1024,7 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
             s_tot[par][wave][0] = s0;
             s_tot[par][wave][1] = s1;
           }
-          __syncthreads();
+          LdsBarrier();
           int32_t f0 = y0, f1 = y1;
           for (int w = 0; w < nwaves; ++w) {
             if (w == wave) {
@@ -1203,7 +1225,39 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
         }
         y0 = tile_out0;
         y1 = tile_out1;
-        if (sync) __syncthreads();  // the next tile / row reads these stores
+        // hand the H value of this segment's last column to the next
+        // segment's first lane
+        if (nseg > 1) {
+          const int32_t last = LastCol<ST, KW>(hw);
+          int32_t got = 0;
+          uint32_t to_sg = 0;
+          bool have = false;
+          if (nwaves > 1) {
+            if (lane == 63) s_bnd[wave] = last;
+            LdsBarrier();
+            if (lane == 0) {
+              if (wave > 0) {
+                got = s_bnd[wave - 1];
+                to_sg = t * static_cast<uint32_t>(nwaves) + wave;
+                have = true;
+              } else if (t + 1 < ntiles) {
+                got = s_bnd[nwaves - 1];
+                to_sg = (t + 1) * static_cast<uint32_t>(nwaves);
+                have = true;
+              }
+            }
+          } else {
+            got = __builtin_amdgcn_readlane(last, 63);
+            if (lane == 0 && t + 1 < ntiles) {
+              to_sg = t + 1;
+              have = true;
+            }
+          }
+          if (have) {
+            bnd[static_cast<uint64_t>(to_sg) * (V + 1) + r] = got;
+            if (single) bprev = got;
+          }
+        }
       }
       fch_prev = fch_r;
     }

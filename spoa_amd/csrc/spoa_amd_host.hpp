@@ -40,6 +40,12 @@ class HipAlignmentEngine : public AlignmentEngine {
   int last_width_ = 0;
 };
 
+// AlignmentEngine::Create's checks and normalisation
+// (src/alignment_engine.cpp:32-66): throws std::invalid_argument with the
+// reference's messages, normalises g/e/q/c in place, returns the subtype.
+AlignmentSubtype CheckCreateParams(AlignmentType type, std::int8_t* g, std::int8_t* e, std::int8_t* q,
+                                   std::int8_t* c);
+
 // Device ordinal for engines created through AlignmentEngine::Create
 // (SPOA_HIP_DEVICE, default 0).
 int DefaultDevice();

@@ -62,7 +62,8 @@ struct WindowSpec {
   std::uint32_t read_len_min, read_len_max;  // 0,0: reads span the whole template
   double err_min, err_max;                   // per-window error rate range
   std::uint32_t revcomp_odd;                 // 1: odd-numbered reads are reverse-complemented
-  std::uint32_t log_uniform_len;             // 1: read length log-uniform in the range
+  std::uint32_t log_uniform_len;             // 1: read length log-uniform in its range (the template's
+                                             // when reads span the template)
 };
 
 struct Window {
@@ -118,7 +119,14 @@ inline std::uint64_t WindowSeed(std::uint64_t seed, std::uint64_t w) {
 inline Window MakeWindow(std::uint64_t seed, const WindowSpec& sp) {
   Rng rng(seed);
   Window w;
-  std::uint32_t tlen = rng.Int(sp.template_min, sp.template_max);
+  std::uint32_t tlen;
+  if (sp.log_uniform_len && sp.read_len_max == 0) {
+    // reads span the template: the template length is the log-uniform one
+    const double lo = static_cast<double>(sp.template_min), hi = static_cast<double>(sp.template_max);
+    tlen = static_cast<std::uint32_t>(lo * std::exp(rng.Real() * std::log(hi / lo)) + 0.5);
+  } else {
+    tlen = rng.Int(sp.template_min, sp.template_max);
+  }
   w.templ.resize(tlen);
   for (auto& b : w.templ) b = rng.Base();
   std::uint32_t nreads = rng.Int(sp.reads_min, sp.reads_max);

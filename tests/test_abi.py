@@ -47,3 +47,15 @@ def test_last_error_is_callable_without_device():
     lib = ctypes.CDLL(LIB)
     lib.spoa_hip_last_error.restype = ctypes.c_char_p
     assert isinstance(lib.spoa_hip_last_error(), bytes)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB) or os.path.exists("/dev/kfd"), reason="needs a host without a GPU")
+def test_product_fails_loudly_without_device():
+    """No CPU fallback: engines and servers refuse to start without a GPU."""
+    import sys
+    sys.path.insert(0, ROOT)
+    import spoa_amd
+    with pytest.raises(ValueError):
+        spoa_amd.AlignmentEngine.create(spoa_amd.NW, 5, -4, -8)
+    with pytest.raises(ValueError):
+        spoa_amd.Server(threads=1)
