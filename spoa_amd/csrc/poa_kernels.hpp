@@ -539,8 +539,17 @@ __device__ __forceinline__ void ProfileWords(const uint32_t (&scp)[K / 4], int32
 // (8 * K * blockDim.x bytes, each lane's words contiguous), used when a row
 // is one tile and the graph has at most 4 distinct bases (the profile is
 // otherwise formed per row).
+// Occupancy target: three waves a SIMD (at most 168 registers a lane) where
+// the row state fits with little spilling -- more waves in flight hide more
+// of the per-row memory latency; wider lanes keep all their registers.
+template <typename ST, int K>
+struct WavesPerSimd {
+  static constexpr int value = (sizeof(ST) == 2 ? K <= 16 : K <= 8) ? 3 : 1;
+};
+
 template <int SUB, typename ST, int K, int MAXT>
-__global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<ST, K>::value)))
+PoaAlignKernel(const KArgs a) {
   constexpr int L = sizeof(ST) == 2 ? 16 : 8;
   constexpr int CPW = Cols<ST>::kPerWord;
   constexpr int KW = K / CPW;  // score words a lane
@@ -568,15 +577,21 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
   const uint32_t G2 = Splat<ST>(g), E2 = Splat<ST>(e), Q2 = Splat<ST>(q), C2 = Splat<ST>(cc);
   // scan constants: the transfer across the lanes of this lane's row up to
   // it (the carry out of the previous row of 16 lanes reaches it), and, for
-  // multi-wave rows, from lane 0 to it (the wave carry)
+  // multi-wave rows, from lane 0 to it (the wave carry); the convex 2x2
+  // matrices live in LDS (read once per row) to spare registers
   int32_t lda = 0, ldc = 0;
-  int32_t lpa[4] = {0, 0, 0, 0}, lpc[4] = {0, 0, 0, 0};
+  __shared__ int4 s_lpa[64], s_lpc[64];
   if constexpr (SUB != 2) {
     lda = ((lane & 15) + 1) * prm.decay;
     if constexpr (MAXT > 64) ldc = lane * prm.decay;
-  } else {
-    LanePow(prm, (lane & 15) + 1, lpa);
-    if constexpr (MAXT > 64) LanePow(prm, lane >= 1 ? lane : 1, lpc);
+  } else if (wave == 0) {
+    int32_t m4[4];
+    LanePow(prm, (lane & 15) + 1, m4);
+    s_lpa[lane] = make_int4(m4[0], m4[1], m4[2], m4[3]);
+    if constexpr (MAXT > 64) {
+      LanePow(prm, lane >= 1 ? lane : 1, m4);
+      s_lpc[lane] = make_int4(m4[0], m4[1], m4[2], m4[3]);
+    }
   }
 
   for (;;) {
@@ -1012,8 +1027,10 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
           const int32_t z0 = lane < 32 ? c0r : (lane < 48 ? c0s : a0);
           const int32_t z1 = lane < 32 ? c1r : (lane < 48 ? c1s : a1);
           if (lane >= 16) {
-            s0 = max(s0, max(lpa[0] + z0, lpa[1] + z1));
-            s1 = max(s1, max(lpa[2] + z0, lpa[3] + z1));
+            const int4 lp = s_lpa[lane];
+            const int32_t n0 = max(lp.x + z0, lp.y + z1), n1 = max(lp.z + z0, lp.w + z1);
+            s0 = max(s0, n0);
+            s1 = max(s1, n1);
           }
         }
         // carry into this wave: the tile carry folded with earlier waves
@@ -1056,8 +1073,10 @@ __global__ void __launch_bounds__(MAXT) PoaAlignKernel(const KArgs a) {
           if constexpr (SUB != 2) {
             x0 = max(x0, w0 + ldc);
           } else {
-            x0 = max(x0, max(lpc[0] + w0, lpc[1] + w1));
-            x1 = max(x1, max(lpc[2] + w0, lpc[3] + w1));
+            const int4 lc = s_lpc[lane];
+            const int32_t n0 = max(lc.x + w0, lc.y + w1), n1 = max(lc.z + w0, lc.w + w1);
+            x0 = max(x0, n0);
+            x1 = max(x1, n1);
           }
         }
 
