@@ -159,6 +159,12 @@ int spoa_hip_set_config(spoa_hip_engine* engine, int32_t width_bits, int32_t bat
  * per-launch times in spoa_hip_stats measure.  Both engines must be on one
  * device, and owner must outlive engine. */
 int spoa_hip_share_compute(spoa_hip_engine* engine, spoa_hip_engine* owner);
+/* No counterpart in the reference.  The engine runs its copies and kernels
+ * on owner's own stream: everything the engines sharing it do runs in order
+ * on that one stream (one hardware queue), e.g. one size class of a serving
+ * loop whose other classes run concurrently on theirs.  Both engines must
+ * be on one device, and owner must outlive engine. */
+int spoa_hip_share_stream(spoa_hip_engine* engine, spoa_hip_engine* owner);
 /* Device memory budget for DP matrices (bytes).  A batch whose matrices do
  * not fit is split into launches that do; a single job that does not fit
  * fails with SPOA_HIP_NO_MEMORY ("insufficient memory!"). */

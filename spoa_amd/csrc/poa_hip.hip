@@ -269,7 +269,8 @@ struct Plan {
 
 struct spoa_hip_engine {
   int device = 0;
-  hipStream_t stream = nullptr;   // copies
+  hipStream_t stream = nullptr;   // the engine's own stream
+  hipStream_t copy = nullptr;     // copies: the own stream, or a shared one
   // kernels: the engine's own stream, or one shared with other engines so
   // that their launches run one after another and never share the device
   hipStream_t compute = nullptr;
@@ -576,7 +577,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   uint8_t* din = static_cast<uint8_t*>(E->d_in.p);
   uint8_t* dout = static_cast<uint8_t*>(E->d_out.p);
   uint32_t* counters = static_cast<uint32_t*>(E->d_counter.p);
-  hipStream_t st = E->stream;
+  hipStream_t st = E->copy;
   hipStream_t cs = E->compute;
   if (!HipOk(hipEventRecord(E->ev0, st), "event") ||
       !HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in") ||
@@ -636,7 +637,7 @@ int RunPassFinish(spoa_hip_engine* E) {
   spoa_hip_job* jobs = pd.jobs;
   const std::vector<Plan>& plans = pd.plans;
   const size_t lo = pd.lo, hi = pd.hi;
-  hipStream_t st = E->stream;
+  hipStream_t st = E->copy;
   const auto t_wait0 = std::chrono::steady_clock::now();
   if (!HipOk(hipStreamSynchronize(st), "kernel")) return SPOA_HIP_DEVICE_ERROR;
   const auto t_wait1 = std::chrono::steady_clock::now();
@@ -837,6 +838,7 @@ int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_e
     return SPOA_HIP_DEVICE_ERROR;
   }
   E->compute = E->stream;
+  E->copy = E->stream;
   *out = E;
   return SPOA_HIP_OK;
 }
@@ -946,7 +948,7 @@ int spoa_hip_align_batch_wait(spoa_hip_engine* E) {
 int spoa_hip_align_batch_ready(spoa_hip_engine* E) {
   if (!E) return -Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null engine");
   if (!E->pending.active) return 1;
-  const hipError_t q = hipStreamQuery(E->stream);
+  const hipError_t q = hipStreamQuery(E->copy);
   if (q == hipErrorNotReady) return 0;
   (void)hipGetLastError();
   return 1;  // finished, or failed: wait reports which
@@ -1006,6 +1008,16 @@ int spoa_hip_share_compute(spoa_hip_engine* E, spoa_hip_engine* owner) {
     owner->compute = owner->compute_own;
   }
   E->compute = owner ? owner->compute : E->stream;
+  return SPOA_HIP_OK;
+}
+
+int spoa_hip_share_stream(spoa_hip_engine* E, spoa_hip_engine* owner) {
+  if (!E) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null engine");
+  if (owner && owner->device != E->device) {
+    return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] engines on different devices cannot share a stream");
+  }
+  if (E->pending.active) (void)RunPassFinish(E);
+  E->compute = E->copy = owner ? owner->stream : E->stream;
   return SPOA_HIP_OK;
 }
 

@@ -702,6 +702,7 @@ PoaAlignKernel(const KArgs a) {
     uint32_t pfH[KW], pfF[KW], pfO[KW];
     bool pf_valid = false;
     uint32_t pf_row = 0;
+    int32_t pf_hm1 = 0;  // with the prefetch: the predecessor's H left of the segment (first lane)
     // one-tile rows: H of the previous row left of this wave's segment
     int32_t bprev = 0;
     if (single && wave > 0) {
@@ -777,6 +778,8 @@ PoaAlignKernel(const KArgs a) {
               hm1 = static_cast<int32_t>(static_cast<ST>(row0_h(static_cast<int32_t>(c0) - 1)));
             } else if (single && p + 1 == r) {
               hm1 = bprev;
+            } else if (fetched) {
+              hm1 = pf_hm1;
             } else {
               hm1 = bnd[static_cast<uint64_t>(t * static_cast<uint32_t>(nwaves) + wave) * (V + 1) + p];
             }
@@ -850,15 +853,37 @@ PoaAlignKernel(const KArgs a) {
           h1[i] = WMax<ST>(h1[i], fv[i]);
           if (SUB == 2) h1[i] = WMax<ST>(h1[i], ov[i]);
         }
-        // prefetch the next row's first predecessor unless it is this row
+        // prefetch the first predecessor of the next (row, tile): the next
+        // tile of this row, or the first tile of the next row unless that
+        // predecessor is this row kept in registers.  Every predecessor
+        // tile was written earlier by these same lanes.
         pf_valid = false;
-        if (single && r < V && nxt.y != r) {
-          const __amdgpu_buffer_rsrc_t rq = RowRsrc(row_ptr(nxt.y), rbytes);
-          BufLoad<KW>(rq, active ? off : rbytes, pfH);
-          if (SUB >= 1) BufLoad<KW>(rq, active ? off + mbytes : rbytes, pfF);
-          if (SUB == 2) BufLoad<KW>(rq, active ? off + 3 * mbytes : rbytes, pfO);
-          pf_valid = true;
-          pf_row = nxt.y;
+        {
+          bool want = false;
+          uint32_t q = 0, qc0 = 0;
+          if (t + 1 < ntiles) {
+            want = true;
+            q = p0;
+            qc0 = c0 + tile_w;
+          } else if (r < V && !(single && nxt.y == r)) {
+            want = true;
+            q = nxt.y;
+            qc0 = static_cast<uint32_t>(tid) * K;
+          }
+          if (want) {
+            const __amdgpu_buffer_rsrc_t rq = RowRsrc(row_ptr(q), rbytes);
+            const bool qa = qc0 < stride;
+            const uint32_t qoff = qc0 * S;
+            BufLoad<KW>(rq, qa ? qoff : rbytes, pfH);
+            if (SUB >= 1) BufLoad<KW>(rq, qa ? qoff + mbytes : rbytes, pfF);
+            if (SUB == 2) BufLoad<KW>(rq, qa ? qoff + 3 * mbytes : rbytes, pfO);
+            if (lane == 0 && qc0 > 0 && q > 0) {
+              const uint32_t qt = t + 1 < ntiles ? t + 1 : 0u;
+              pf_hm1 = bnd[static_cast<uint64_t>(qt * static_cast<uint32_t>(nwaves) + wave) * (V + 1) + q];
+            }
+            pf_valid = true;
+            pf_row = q;
+          }
         }
         if (SUB >= 1 && active) {
           BufStore<KW>(rrow, off + mbytes, fv);

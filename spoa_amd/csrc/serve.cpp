@@ -31,7 +31,8 @@ bool ServeParams::operator<(const ServeParams& o) const {
          std::make_tuple(static_cast<int>(o.type), o.m, o.n, o.g, o.e, o.q, o.c);
 }
 
-// One engine (device buffers and stream) per lane for a parameter set.
+// One engine (device buffers) per lane for a parameter set, on its lane's
+// stream.
 struct Server::Engines {
   std::vector<std::unique_ptr<spoa::AlignmentEngine>> eng;
   std::vector<spoa_hip_engine*> dev;
@@ -74,7 +75,11 @@ Server::Server(const ServeOptions& opt) : opt_(opt), pool_(opt.threads), t0_(Clo
     const std::uint32_t sets = std::max<std::uint32_t>(1u, opt_.max_param_sets);
     engine_budget_ = static_cast<std::uint64_t>(total / (static_cast<double>(nl) * sets));
   }
-  for (std::size_t h = 0; h < nl; ++h) lanes_.emplace_back(new Lane());
+  for (std::size_t h = 0; h < nl; ++h) {
+    lanes_.emplace_back(new Lane());
+    lane_owner_.push_back(spoa::AlignmentEngine::Create(spoa::AlignmentType::kNW, 1, -1, -1));
+    spoa_hip_set_memory_budget(dynamic_cast<spoa::HipAlignmentEngine*>(lane_owner_.back().get())->device(), 0);
+  }
   pending_.resize(nl);
   thread_ = std::thread([this] { Loop(); });
 }
@@ -161,6 +166,8 @@ Server::Engines* Server::EnginesFor(const ServeParams& p) {
     en->eng.push_back(spoa::AlignmentEngine::Create(p.type, p.m, p.n, p.g, p.e, p.q, p.c));
     en->dev.push_back(dynamic_cast<spoa::HipAlignmentEngine*>(en->eng.back().get())->device());
     spoa_hip_set_memory_budget(en->dev.back(), engine_budget_);
+    auto* owner = dynamic_cast<spoa::HipAlignmentEngine*>(lane_owner_[h].get())->device();
+    if (spoa_hip_share_stream(en->dev.back(), owner) != SPOA_HIP_OK) throw std::invalid_argument(spoa_hip_last_error());
   }
   Engines* raw = en.get();
   engines_[p] = std::move(en);

@@ -7,10 +7,10 @@
 // launch per scoring parameter set aligns the next read of every job of the
 // lane, then the host merges the alignments into the jobs' graphs (thread
 // pool), finishes completed jobs (consensus, MSA) and admits waiting jobs of
-// its class into the free slots.  Lanes have their own engines and streams
-// and a scheduler thread polls them, so a lane of short jobs cycles at its
-// own pace while a lane of long alignments is still on the device, and
-// their launches share the GPU.  Device memory is admitted per round: a
+// its class into the free slots.  Each lane runs on its own stream (its own
+// hardware queue) and a scheduler thread polls the lanes, so a lane of
+// short jobs cycles at its own pace while a lane of long alignments is
+// still on the device, and their launches share the GPU.  Device memory is admitted per round: a
 // job's next alignment joins a round only while the DP-matrix footprints of
 // the round's alignments fit the engine's budget (spoa_hip_job_footprint);
 // otherwise it waits for the next round.
@@ -121,7 +121,10 @@ class Server {
   bool stop_ = false;
   ServeStats stats_;
 
-  // scheduler-thread state
+  // scheduler-thread state.  Every lane owns one stream (held by an engine
+  // that never aligns, declared first so that it is destroyed last); the
+  // engines of the lane run their copies and kernels on it in order.
+  std::vector<std::unique_ptr<spoa::AlignmentEngine>> lane_owner_;
   std::map<ServeParams, std::unique_ptr<Engines>> engines_;
   std::uint64_t engine_budget_ = 0;
   std::vector<std::unique_ptr<Lane>> lanes_;
