@@ -246,6 +246,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-call", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch configurations)")
+    ap.add_argument("--tune-rounds", type=int, default=8)
+    ap.add_argument("--tune-cache", default=None, help="autotuner cache file (JSON)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -274,6 +277,10 @@ def main():
     windows = spoa_amd.config_windows(args.config, args.seed, n_windows)
 
     ctx = spoa_amd.BatchContext(*params, strand=strand, msa=True, threads=threads)
+    tuned = None
+    if not args.no_tune:
+        # launch configuration per score width from the autotuner (untimed)
+        tuned = ctx.autotune(windows, rounds=args.tune_rounds, cache=args.tune_cache)
 
     def step():
         return ctx.run(windows)
@@ -339,6 +346,7 @@ def main():
         "breakdown_ms_per_step": {k: last[k] for k in ("wall_ms", "align_ms", "kernel_ms", "h2d_ms",
                                                        "d2h_ms", "pack_ms", "alloc_ms", "wait_ms",
                                                        "unpack_ms", "graph_ms", "finish_ms")},
+        "autotuned": tuned,
         "per_call": per_call,
         "batched_vs_per_call": (wps / per_call["windows_per_s"]) if per_call else None,
         "cpu_baseline": cpu,

@@ -76,6 +76,7 @@ typedef struct spoa_batch_opts {
   int32_t threads; /* host threads, 0 = all */
   int32_t device;
   int32_t overlap; /* batched form: the two halves' launches may run on the device at once */
+  int32_t graph;   /* batched form: every pass launched as a captured HIP graph */
 } spoa_batch_opts;
 
 typedef struct spoa_batch_stats {
@@ -83,6 +84,7 @@ typedef struct spoa_batch_stats {
   double pack_ms, alloc_ms, wait_ms, unpack_ms;
   uint64_t cells, dp_bytes, align_calls;
   uint32_t launches, rounds;
+  double kernel_ms16, kernel_ms32; /* kernel_ms of the int16 / int32 launches */
 } spoa_batch_stats;
 
 /* Runs every window (reads [win_read_off[w], win_read_off[w+1]); read k is
@@ -104,6 +106,15 @@ spoa_hip_engine* spoa_batch_context_device(void* ctx);
  * on both engines); cols_per_lane 0 clears that width's overrides. */
 int spoa_batch_context_set_config(void* ctx, int32_t width_bits, uint32_t max_width, int32_t cols_per_lane,
                                   int32_t waves);
+/* The autotuner (no counterpart in the reference): tries launch
+ * configurations on the given windows (their first `rounds` rounds), keeps
+ * the fastest per score width and installs it on the context; cache_path
+ * (or NULL) names a JSON cache.  Writes per width {width, cols_per_lane,
+ * waves, from_cache} into out (4 ints each) and the chosen configuration's
+ * device ms into out_ms; returns the number of widths, -1 on error. */
+int spoa_batch_context_autotune(void* ctx, uint32_t n_windows, const uint32_t* win_read_off, const uint64_t* read_off,
+                                const char* seqs, uint32_t rounds, const char* cache_path, int32_t* out,
+                                double* out_ms, uint32_t cap);
 void* spoa_batch_context_run(void* ctx, uint32_t n_windows, const uint32_t* win_read_off, const uint64_t* read_off,
                              const char* seqs, const char* quals);
 int spoa_batch_get_stats(void* result, spoa_batch_stats* out);

@@ -135,6 +135,8 @@ typedef struct spoa_hip_stats {
   uint32_t jobs;
   uint64_t cells;
   uint64_t dp_bytes;
+  double kernel_ms16;  /* kernel_ms of the int16-score launches */
+  double kernel_ms32;  /* kernel_ms of the int32-score launches */
 } spoa_hip_stats;
 int spoa_hip_last_stats(spoa_hip_engine* engine, spoa_hip_stats* out);
 
@@ -165,6 +167,11 @@ int spoa_hip_share_compute(spoa_hip_engine* engine, spoa_hip_engine* owner);
  * loop whose other classes run concurrently on theirs.  Both engines must
  * be on one device, and owner must outlive engine. */
 int spoa_hip_share_stream(spoa_hip_engine* engine, spoa_hip_engine* owner);
+/* No counterpart in the reference.  on != 0: every pass of align_batch is
+ * captured into a HIP graph (copy in, counter reset, the launches, copy
+ * out) and launched as one graph on the compute stream; the stats then
+ * time the whole graph as one launch (copies included).  Default off. */
+int spoa_hip_set_graph_mode(spoa_hip_engine* engine, int32_t on);
 /* Device memory budget for DP matrices (bytes).  A batch whose matrices do
  * not fit is split into launches that do; a single job that does not fit
  * fails with SPOA_HIP_NO_MEMORY ("insufficient memory!"). */

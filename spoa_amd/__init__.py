@@ -43,12 +43,14 @@ class HipStats(ctypes.Structure):
                 ("alloc_ms", ctypes.c_double), ("wait_ms", ctypes.c_double),
                 ("unpack_ms", ctypes.c_double),
                 ("launches", ctypes.c_uint32), ("jobs", ctypes.c_uint32),
-                ("cells", ctypes.c_uint64), ("dp_bytes", ctypes.c_uint64)]
+                ("cells", ctypes.c_uint64), ("dp_bytes", ctypes.c_uint64),
+                ("kernel_ms16", ctypes.c_double), ("kernel_ms32", ctypes.c_double)]
 
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int32) for k in
-                ("type", "m", "n", "g", "e", "q", "c", "strand", "msa", "threads", "device", "overlap")]
+                ("type", "m", "n", "g", "e", "q", "c", "strand", "msa", "threads", "device", "overlap",
+                 "graph")]
 
 
 class BatchStats(ctypes.Structure):
@@ -59,7 +61,8 @@ class BatchStats(ctypes.Structure):
                 ("alloc_ms", ctypes.c_double), ("wait_ms", ctypes.c_double),
                 ("unpack_ms", ctypes.c_double), ("cells", ctypes.c_uint64),
                 ("dp_bytes", ctypes.c_uint64), ("align_calls", ctypes.c_uint64),
-                ("launches", ctypes.c_uint32), ("rounds", ctypes.c_uint32)]
+                ("launches", ctypes.c_uint32), ("rounds", ctypes.c_uint32),
+                ("kernel_ms16", ctypes.c_double), ("kernel_ms32", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -134,6 +137,10 @@ def lib():
     L.spoa_batch_context_set_config.restype = ctypes.c_int
     L.spoa_batch_context_set_config.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int32,
                                                 ctypes.c_int32]
+    L.spoa_batch_context_autotune.restype = ctypes.c_int
+    L.spoa_batch_context_autotune.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_uint32]
     L.spoa_batch_context_run.restype = ctypes.c_void_p
     L.spoa_batch_context_run.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_char_p, ctypes.c_char_p]
@@ -447,13 +454,13 @@ class WindowResults:
             self._h = None
 
 
-def _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap=False):
+def _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap=False, graph=False):
     if e is None:
         e = g
     if q is None:
         q, c = g, e
     return BatchOpts(type=type, m=m, n=n, g=g, e=e, q=q, c=c, strand=int(strand), msa=int(msa),
-                     threads=threads, device=device, overlap=int(overlap))
+                     threads=threads, device=device, overlap=int(overlap), graph=int(graph))
 
 
 class BatchContext:
@@ -461,8 +468,8 @@ class BatchContext:
     between runs) for one set of scoring options."""
 
     def __init__(self, type, m, n, g, e=None, q=None, c=None, strand=False, msa=True, threads=0, device=0,
-                 overlap=False):
-        self.opts = _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap)
+                 overlap=False, graph=False):
+        self.opts = _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap, graph)
         self._h = lib().spoa_batch_context_new(ctypes.byref(self.opts))
         if not self._h:
             raise ValueError(_error())
@@ -475,6 +482,22 @@ class BatchContext:
     @property
     def device(self):
         return lib().spoa_batch_context_device(self._h)
+
+    def autotune(self, windows, rounds=8, cache=None):
+        """Picks the launch configuration (columns per lane, waves) of each
+        score width by timing candidates on these windows (their first
+        rounds); returns [{width, cols_per_lane, waves, cached, kernel_ms}]."""
+        if not isinstance(windows, Windows):
+            windows = Windows.from_lists(windows)
+        out = np.zeros(8, dtype=np.int32)
+        ms = np.zeros(2, dtype=np.float64)
+        n = lib().spoa_batch_context_autotune(self._h, windows.n_windows, windows.win_read_off.ctypes.data,
+                                              windows.read_off.ctypes.data, windows.seqs, rounds,
+                                              _b(cache) if cache else None, out.ctypes.data, ms.ctypes.data, 2)
+        if n < 0:
+            raise ValueError(_error())
+        return [{"width": int(out[4 * i]), "cols_per_lane": int(out[4 * i + 1]), "waves": int(out[4 * i + 2]),
+                 "cached": bool(out[4 * i + 3]), "kernel_ms": float(ms[i])} for i in range(n)]
 
     def set_config(self, width_bits, cols_per_lane, waves, max_width=0xFFFFFFFF):
         """Launch configuration of the batched launches for rows up to

@@ -5,6 +5,9 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <array>
+#include <cstdio>
+#include <map>
 #include <stdexcept>
 
 #include "poa_graph.hpp"
@@ -131,6 +134,9 @@ BatchContext::BatchContext(const BatchOptions& opt) : opt_(opt), pool_(opt.threa
   }
   if (!opt.overlap && spoa_hip_share_compute(dev[1], dev[0]) != SPOA_HIP_OK) {
     throw std::invalid_argument(spoa_hip_last_error());
+  }
+  if (opt.graph) {
+    for (auto* d : dev) spoa_hip_set_graph_mode(d, 1);
   }
 }
 
@@ -262,6 +268,8 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
     spoa_hip_stats hs{};
     spoa_hip_last_stats(hf.dev, &hs);
     st.kernel_ms += hs.kernel_ms;
+    st.kernel_ms16 += hs.kernel_ms16;
+    st.kernel_ms32 += hs.kernel_ms32;
     st.h2d_ms += hs.h2d_ms;
     st.d2h_ms += hs.d2h_ms;
     st.pack_ms += hs.host_ms;
@@ -327,6 +335,7 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
   };
 
   // Pipeline: the device aligns one half while the host merges the other.
+  if (opt.max_rounds > 0) max_reads = std::min(max_reads, opt.max_rounds);
   for (std::uint32_t r = 0; r < max_reads; ++r) {
     for (int h = 0; h < 2; ++h) {
       Half& hf = half[h];
@@ -347,6 +356,11 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
   }
 
   const auto tf = Clock::now();
+  if (opt.max_rounds > 0) {  // a short run: no consensus
+    st.wall_ms = Ms(t0, Clock::now());
+    if (stats) *stats = st;
+    return out;
+  }
   pool.For(W, [&](std::size_t w) {
     out[w].consensus = graphs[w].GenerateConsensus();
     if (opt.msa) out[w].msa = graphs[w].GenerateMultipleSequenceAlignment();
@@ -416,6 +430,141 @@ std::vector<WindowResult> RunPerCall(const WindowSet& ws, const BatchOptions& op
   st.wall_ms = Ms(t0, Clock::now());
   if (stats) *stats = st;
   return out;
+}
+
+// ------------------------------------------------------------- autotune --
+
+namespace {
+
+std::string TuneKey(const BatchOptions& o, int width, std::uint32_t max_len, std::uint32_t n_windows) {
+  int subtype = 0;
+  std::int8_t g = o.g, e = o.e, q = o.q, c = o.c;
+  subtype = static_cast<int>(spoa::CheckCreateParams(o.type, &g, &e, &q, &c));
+  int lb = 0, jb = 0;
+  while ((1u << lb) < max_len) ++lb;
+  while ((1u << jb) < n_windows) ++jb;
+  return "sub" + std::to_string(subtype) + "_type" + std::to_string(static_cast<int>(o.type)) + "_w" +
+         std::to_string(width) + "_len" + std::to_string(lb) + "_win" + std::to_string(jb);
+}
+
+// The cache: one line per entry, `"key": [k, waves, kernel_ms]`, inside braces.
+std::map<std::string, std::array<double, 3>> LoadTuneCache(const std::string& path) {
+  std::map<std::string, std::array<double, 3>> out;
+  if (path.empty()) return out;
+  std::FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return out;
+  char line[512];
+  while (std::fgets(line, sizeof(line), f)) {
+    char key[256];
+    double k = 0, w = 0, ms = 0;
+    if (std::sscanf(line, " \"%255[^\"]\": [%lf, %lf, %lf]", key, &k, &w, &ms) == 4) out[key] = {k, w, ms};
+  }
+  std::fclose(f);
+  return out;
+}
+
+void SaveTuneCache(const std::string& path, const std::map<std::string, std::array<double, 3>>& c) {
+  if (path.empty()) return;
+  const std::string tmp = path + ".tmp";
+  std::FILE* f = std::fopen(tmp.c_str(), "w");
+  if (!f) return;
+  std::fprintf(f, "{\n");
+  std::size_t i = 0;
+  for (const auto& kv : c) {
+    std::fprintf(f, "  \"%s\": [%d, %d, %.3f]%s\n", kv.first.c_str(), static_cast<int>(kv.second[0]),
+                 static_cast<int>(kv.second[1]), kv.second[2], ++i < c.size() ? "," : "");
+  }
+  std::fprintf(f, "}\n");
+  std::fclose(f);
+  std::rename(tmp.c_str(), path.c_str());
+}
+
+}  // namespace
+
+std::vector<TuneResult> Autotune(const WindowSet& ws, BatchContext& ctx, std::uint32_t rounds,
+                                 const std::string& cache_path) {
+  BatchOptions& opt = ctx.mutable_options();
+  spoa_hip_engine* dev[2];
+  for (int h = 0; h < 2; ++h) {
+    auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(ctx.engine(h));
+    if (!hip) throw std::invalid_argument("[spoa::hip] the autotuner needs the HIP engine");
+    dev[h] = hip->device();
+  }
+  std::int8_t g = opt.g, e = opt.e, q = opt.q, c = opt.c;
+  const bool convex = spoa::CheckCreateParams(opt.type, &g, &e, &q, &c) == spoa::AlignmentSubtype::kConvex;
+  std::uint32_t max_len = 0;
+  for (std::uint64_t k = 0; k < ws.win_read_off[ws.n_windows]; ++k) {
+    max_len = std::max<std::uint32_t>(max_len, static_cast<std::uint32_t>(ws.read_off[k + 1] - ws.read_off[k]));
+  }
+  auto cache = LoadTuneCache(cache_path);
+  auto set = [&](int width, int k, int waves) {
+    for (int h = 0; h < 2; ++h) {
+      if (spoa_hip_set_config(dev[h], width, 1, 0xffffffffu, k, waves) != SPOA_HIP_OK) {
+        throw std::invalid_argument(spoa_hip_last_error());
+      }
+    }
+  };
+  const std::uint32_t saved_rounds = opt.max_rounds;
+  opt.max_rounds = std::max<std::uint32_t>(2u, rounds);
+  std::vector<TuneResult> results;
+  try {
+    for (int width : {16, 32}) {
+      TuneResult tr;
+      tr.width = width;
+      const std::string key = TuneKey(opt, width, max_len, ws.n_windows);
+      auto it = cache.find(key);
+      if (it != cache.end()) {
+        tr.k = static_cast<int>(it->second[0]);
+        tr.waves = static_cast<int>(it->second[1]);
+        tr.cached = true;
+        set(width, tr.k, tr.waves);
+        results.push_back(tr);
+        continue;
+      }
+      // candidates: lanes of 8 / 16 / 32 columns (convex int16 at least 16,
+      // int32 at most 16), as many waves as a row needs and twice that
+      const int L = width == 16 ? 16 : 8;
+      const std::uint32_t W = (max_len + L - 1) / L * L;
+      std::vector<std::pair<int, int>> cands;
+      for (int k : {8, 16, 32}) {
+        if (width == 16 && convex && k < 16) continue;
+        if (width == 32 && k > 16) continue;
+        const int need = static_cast<int>(std::min<std::uint32_t>(8u, (W + 64u * k - 1) / (64u * k)));
+        for (int wv : {need, 2 * need}) {
+          if (wv < 1 || wv > 8 || (k == 32 && wv != 1)) continue;
+          if (std::find(cands.begin(), cands.end(), std::make_pair(k, wv)) == cands.end()) cands.push_back({k, wv});
+        }
+      }
+      double best = -1;
+      for (const auto& cd : cands) {
+        set(width, cd.first, cd.second);
+        BatchStats st;
+        (void)RunBatched(ws, ctx, &st);
+        const double ms = width == 16 ? st.kernel_ms16 : st.kernel_ms32;
+        tr.tried.push_back({cd, ms});
+        if (ms <= 0) continue;  // no job of this width
+        if (best < 0 || ms < best) {
+          best = ms;
+          tr.k = cd.first;
+          tr.waves = cd.second;
+          tr.kernel_ms = ms;
+        }
+      }
+      if (best < 0) {
+        set(width, 0, 0);  // no job of this width: keep the defaults
+      } else {
+        set(width, tr.k, tr.waves);
+        cache[key] = {static_cast<double>(tr.k), static_cast<double>(tr.waves), tr.kernel_ms};
+      }
+      results.push_back(tr);
+    }
+  } catch (...) {
+    opt.max_rounds = saved_rounds;
+    throw;
+  }
+  opt.max_rounds = saved_rounds;
+  SaveTuneCache(cache_path, cache);
+  return results;
 }
 
 }  // namespace spoa_amd

@@ -59,6 +59,8 @@ struct BatchOptions {
   bool strand = false;     // align both strands, keep the better (src/main.cpp:285-302)
   bool msa = true;         // also build the MSA of every window
   bool overlap = false;    // the two halves' launches may share the device (else they take turns)
+  std::uint32_t max_rounds = 0;  // stop after this many rounds (0: all; the autotuner's short runs)
+  bool graph = false;      // launch every pass as a captured HIP graph
   unsigned threads = 0;    // host threads (0: all hardware threads)
   int device = 0;
 };
@@ -73,6 +75,7 @@ struct BatchStats {
   double wall_ms = 0;        // whole run
   double align_ms = 0;       // inside the alignment calls (pack + copies + kernels + unpack)
   double kernel_ms = 0;      // alignment kernels (HIP events)
+  double kernel_ms16 = 0, kernel_ms32 = 0;  // of the int16 / int32 launches
   double h2d_ms = 0, d2h_ms = 0;
   double graph_ms = 0;       // AddAlignment + topological sort
   double finish_ms = 0;      // consensus + MSA
@@ -93,6 +96,7 @@ class BatchContext {
   explicit BatchContext(const BatchOptions& opt);
   const BatchOptions& options() const { return opt_; }
   spoa::AlignmentEngine* engine(int half = 0) { return engines_[half].get(); }
+  BatchOptions& mutable_options() { return opt_; }
   ThreadPool& pool() { return pool_; }
 
  private:
@@ -105,6 +109,24 @@ class BatchContext {
 // failure.
 std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, BatchStats* stats);
 std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats);
+
+// Autotuner of the batched launches: for each score width, tries launch
+// configurations (columns per lane, waves per workgroup) on the windows
+// (the first `rounds` rounds of the batched form, every candidate on the
+// same bytes), keeps the one with the least device time of that width's
+// launches and installs it on the context's engines.  Results are cached
+// in cache_path (a JSON file; empty: no cache) under a key of the scoring
+// subtype, alignment type, width, read-length and batch-size buckets, so
+// a later run with the same shape installs them without measuring.
+struct TuneResult {
+  int width = 0;
+  int k = 0, waves = 0;
+  double kernel_ms = 0;      // of the chosen configuration (0: from the cache)
+  bool cached = false;
+  std::vector<std::pair<std::pair<int, int>, double>> tried;  // ((k, waves), kernel ms)
+};
+std::vector<TuneResult> Autotune(const WindowSet& ws, BatchContext& ctx, std::uint32_t rounds,
+                                 const std::string& cache_path);
 
 // Plain per-call form: AlignmentEngine::Create(...)->Align for every read
 // of every window, in order, on the calling thread.

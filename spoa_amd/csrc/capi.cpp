@@ -57,6 +57,7 @@ spoa_amd::BatchOptions Options(const spoa_batch_opts* o) {
   b.threads = o->threads > 0 ? static_cast<unsigned>(o->threads) : 0u;
   b.device = o->device;
   b.overlap = o->overlap != 0;
+  b.graph = o->graph != 0;
   return b;
 }
 
@@ -283,6 +284,34 @@ int spoa_batch_context_set_config(void* ctx, int32_t width_bits, uint32_t max_wi
   return 0;
 }
 
+int spoa_batch_context_autotune(void* ctx, uint32_t n_windows, const uint32_t* win_read_off, const uint64_t* read_off,
+                                const char* seqs, uint32_t rounds, const char* cache_path, int32_t* out,
+                                double* out_ms, uint32_t cap) {
+  try {
+    spoa_amd::WindowSet ws;
+    ws.n_windows = n_windows;
+    ws.win_read_off = win_read_off;
+    ws.read_off = read_off;
+    ws.seqs = seqs;
+    auto res = spoa_amd::Autotune(ws, *static_cast<spoa_amd::BatchContext*>(ctx), rounds,
+                                  cache_path ? std::string(cache_path) : std::string());
+    uint32_t n = 0;
+    for (const auto& r : res) {
+      if (n >= cap) break;
+      out[4 * n + 0] = r.width;
+      out[4 * n + 1] = r.k;
+      out[4 * n + 2] = r.waves;
+      out[4 * n + 3] = r.cached ? 1 : 0;
+      out_ms[n] = r.kernel_ms;
+      ++n;
+    }
+    return static_cast<int>(n);
+  } catch (const std::exception& ex) {
+    Error(ex);
+    return -1;
+  }
+}
+
 void* spoa_batch_context_run(void* ctx, uint32_t n_windows, const uint32_t* win_read_off, const uint64_t* read_off,
                              const char* seqs, const char* quals) {
   try {
@@ -319,6 +348,8 @@ int spoa_batch_get_stats(void* h, spoa_batch_stats* out) {
   out->align_calls = s.align_calls;
   out->launches = s.launches;
   out->rounds = s.rounds;
+  out->kernel_ms16 = s.kernel_ms16;
+  out->kernel_ms32 = s.kernel_ms32;
   return 0;
 }
 

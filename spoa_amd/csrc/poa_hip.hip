@@ -279,6 +279,13 @@ struct spoa_hip_engine {
   hipEvent_t ev_in = nullptr, ev_kdone = nullptr;  // copy -> kernels -> copy ordering
   std::vector<hipEvent_t> ev_ks, ev_ke;            // start / end of every launch of a pass
   size_t pass_launches = 0;
+  // graph mode: every pass's launch sequence is captured into a HIP graph
+  // and launched as one unit on the compute stream
+  bool graph_mode = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_pass = false;  // the pending pass ran as a graph
+  std::vector<int> pass_width;  // score width of every launch of the pass
   spoa_hip_params prm{};
   int num_cus = 256;
   uint64_t budget = 0;
@@ -579,19 +586,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   uint32_t* counters = static_cast<uint32_t*>(E->d_counter.p);
   hipStream_t st = E->copy;
   hipStream_t cs = E->compute;
-  if (!HipOk(hipEventRecord(E->ev0, st), "event") ||
-      !HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in") ||
-      !HipOk(hipEventRecord(E->ev1, st), "event")) {
-    return SPOA_HIP_DEVICE_ERROR;
-  }
-  if (cs != st && (!HipOk(hipEventRecord(E->ev_in, st), "event") ||
-                   !HipOk(hipStreamWaitEvent(cs, E->ev_in, 0), "stream wait"))) {
-    return SPOA_HIP_DEVICE_ERROR;
-  }
-  spoa_hip_dev::ResetCounters<<<1, 256, 0, cs>>>(counters, static_cast<uint32_t>(groups.size()));
-  if (!HipOk(hipGetLastError(), "counter reset")) return SPOA_HIP_DEVICE_ERROR;
-  // the groups run one after another on the compute stream, each timed
-  for (size_t gi = 0; gi < groups.size(); ++gi) {
+  auto args_of = [&](size_t gi) {
     const Group& gp = groups[gi];
     KArgs args{};
     args.prm = gp.prm;
@@ -602,22 +597,75 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     args.seqs = din + off_seq;
     args.pairs = reinterpret_cast<int32_t*>(dout + off_pairs);
     args.results = reinterpret_cast<int32_t*>(dout + off_res) + 4 * gp.a;
-    if (!HipOk(hipEventRecord(E->ev_ks[gi], cs), "event")) return SPOA_HIP_DEVICE_ERROR;
-    gp.fn<<<gp.grid, gp.threads, gp.lds, cs>>>(args);
-    if (!HipOk(hipGetLastError(), "kernel launch") || !HipOk(hipEventRecord(E->ev_ke[gi], cs), "event")) {
+    return args;
+  };
+  E->pass_width.clear();
+  for (const Group& gp : groups) E->pass_width.push_back(plans[lo + gp.a].width);
+  E->stats.launches += static_cast<uint32_t>(groups.size());
+  E->graph_pass = E->graph_mode;
+  if (E->graph_mode) {
+    // capture copy in, counter reset (a kernel, not a memset node), the
+    // group launches and copy out on the copy stream, then launch the graph
+    // on the compute stream; its device time is timed as one unit
+    if (!HipOk(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture")) {
       return SPOA_HIP_DEVICE_ERROR;
     }
-    E->stats.launches += 1;
-  }
-  E->pass_launches = groups.size();
-  if (cs != st && (!HipOk(hipEventRecord(E->ev_kdone, cs), "event") ||
-                   !HipOk(hipStreamWaitEvent(st, E->ev_kdone, 0), "stream wait"))) {
-    return SPOA_HIP_DEVICE_ERROR;
-  }
-  if (!HipOk(hipEventRecord(E->ev2, st), "event") ||
-      !HipOk(hipMemcpyAsync(E->h_out.p, E->d_out.p, out_bytes, hipMemcpyDeviceToHost, st), "copy out") ||
-      !HipOk(hipEventRecord(E->ev3, st), "event")) {
-    return SPOA_HIP_DEVICE_ERROR;
+    bool ok = HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in");
+    if (ok) {
+      spoa_hip_dev::ResetCounters<<<1, 256, 0, st>>>(counters, static_cast<uint32_t>(groups.size()));
+      ok = HipOk(hipGetLastError(), "counter reset");
+    }
+    for (size_t gi = 0; ok && gi < groups.size(); ++gi) {
+      const Group& gp = groups[gi];
+      gp.fn<<<gp.grid, gp.threads, gp.lds, st>>>(args_of(gi));
+      ok = HipOk(hipGetLastError(), "kernel launch");
+    }
+    if (ok) ok = HipOk(hipMemcpyAsync(E->h_out.p, E->d_out.p, out_bytes, hipMemcpyDeviceToHost, st), "copy out");
+    hipGraph_t gr = nullptr;
+    const bool ended = HipOk(hipStreamEndCapture(st, &gr), "graph capture");
+    if (!ok || !ended) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+    E->graph = gr;
+    if (!HipOk(hipGraphInstantiate(&E->graph_exec, gr, nullptr, nullptr, 0), "graph instantiate") ||
+        !HipOk(hipEventRecord(E->ev_ks[0], cs), "event") || !HipOk(hipGraphLaunch(E->graph_exec, cs), "graph launch") ||
+        !HipOk(hipEventRecord(E->ev_ke[0], cs), "event") || !HipOk(hipEventRecord(E->ev_kdone, cs), "event")) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+    E->pass_launches = 1;
+    E->pass_width.assign(1, plans[lo].width);
+  } else {
+    if (!HipOk(hipEventRecord(E->ev0, st), "event") ||
+        !HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in") ||
+        !HipOk(hipEventRecord(E->ev1, st), "event")) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+    if (cs != st && (!HipOk(hipEventRecord(E->ev_in, st), "event") ||
+                     !HipOk(hipStreamWaitEvent(cs, E->ev_in, 0), "stream wait"))) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+    spoa_hip_dev::ResetCounters<<<1, 256, 0, cs>>>(counters, static_cast<uint32_t>(groups.size()));
+    if (!HipOk(hipGetLastError(), "counter reset")) return SPOA_HIP_DEVICE_ERROR;
+    // the groups run one after another on the compute stream, each timed
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      const Group& gp = groups[gi];
+      if (!HipOk(hipEventRecord(E->ev_ks[gi], cs), "event")) return SPOA_HIP_DEVICE_ERROR;
+      gp.fn<<<gp.grid, gp.threads, gp.lds, cs>>>(args_of(gi));
+      if (!HipOk(hipGetLastError(), "kernel launch") || !HipOk(hipEventRecord(E->ev_ke[gi], cs), "event")) {
+        return SPOA_HIP_DEVICE_ERROR;
+      }
+    }
+    E->pass_launches = groups.size();
+    if (cs != st && (!HipOk(hipEventRecord(E->ev_kdone, cs), "event") ||
+                     !HipOk(hipStreamWaitEvent(st, E->ev_kdone, 0), "stream wait"))) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+    if (!HipOk(hipEventRecord(E->ev2, st), "event") ||
+        !HipOk(hipMemcpyAsync(E->h_out.p, E->d_out.p, out_bytes, hipMemcpyDeviceToHost, st), "copy out") ||
+        !HipOk(hipEventRecord(E->ev3, st), "event")) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
   }
   pd.active = true;
   pd.jobs = jobs;
@@ -639,18 +687,36 @@ int RunPassFinish(spoa_hip_engine* E) {
   const size_t lo = pd.lo, hi = pd.hi;
   hipStream_t st = E->copy;
   const auto t_wait0 = std::chrono::steady_clock::now();
-  if (!HipOk(hipStreamSynchronize(st), "kernel")) return SPOA_HIP_DEVICE_ERROR;
+  if (E->graph_pass) {
+    const bool ok = HipOk(hipEventSynchronize(E->ev_kdone), "graph");
+    if (E->graph_exec) (void)hipGraphExecDestroy(E->graph_exec);
+    if (E->graph) (void)hipGraphDestroy(E->graph);
+    E->graph_exec = nullptr;
+    E->graph = nullptr;
+    if (!ok) return SPOA_HIP_DEVICE_ERROR;
+  } else if (!HipOk(hipStreamSynchronize(st), "kernel")) {
+    return SPOA_HIP_DEVICE_ERROR;
+  }
   const auto t_wait1 = std::chrono::steady_clock::now();
   E->stats.wait_ms += std::chrono::duration<double, std::milli>(t_wait1 - t_wait0).count();
-  float ms_in = 0, ms_out = 0;
-  (void)hipEventElapsedTime(&ms_in, E->ev0, E->ev1);
-  (void)hipEventElapsedTime(&ms_out, E->ev2, E->ev3);
-  E->stats.h2d_ms += ms_in;
-  E->stats.d2h_ms += ms_out;
+  if (!E->graph_pass) {
+    float ms_in = 0, ms_out = 0;
+    (void)hipEventElapsedTime(&ms_in, E->ev0, E->ev1);
+    (void)hipEventElapsedTime(&ms_out, E->ev2, E->ev3);
+    E->stats.h2d_ms += ms_in;
+    E->stats.d2h_ms += ms_out;
+  }
   for (size_t gi = 0; gi < E->pass_launches; ++gi) {
     float ms_k = 0;
     (void)hipEventElapsedTime(&ms_k, E->ev_ks[gi], E->ev_ke[gi]);
     E->stats.kernel_ms += ms_k;
+    if (gi < E->pass_width.size()) {
+      if (E->pass_width[gi] == 16) {
+        E->stats.kernel_ms16 += ms_k;
+      } else {
+        E->stats.kernel_ms32 += ms_k;
+      }
+    }
   }
 
   // ---- unpack: pairs come back in traceback order
@@ -859,6 +925,8 @@ int spoa_hip_engine_destroy(spoa_hip_engine* E) {
   if (E->ev1) (void)hipEventDestroy(E->ev1);
   if (E->ev2) (void)hipEventDestroy(E->ev2);
   if (E->ev3) (void)hipEventDestroy(E->ev3);
+  if (E->graph_exec) (void)hipGraphExecDestroy(E->graph_exec);
+  if (E->graph) (void)hipGraphDestroy(E->graph);
   if (E->ev_in) (void)hipEventDestroy(E->ev_in);
   if (E->ev_kdone) (void)hipEventDestroy(E->ev_kdone);
   for (auto ev : E->ev_ks) (void)hipEventDestroy(ev);
@@ -948,7 +1016,7 @@ int spoa_hip_align_batch_wait(spoa_hip_engine* E) {
 int spoa_hip_align_batch_ready(spoa_hip_engine* E) {
   if (!E) return -Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null engine");
   if (!E->pending.active) return 1;
-  const hipError_t q = hipStreamQuery(E->copy);
+  const hipError_t q = E->graph_pass ? hipEventQuery(E->ev_kdone) : hipStreamQuery(E->copy);
   if (q == hipErrorNotReady) return 0;
   (void)hipGetLastError();
   return 1;  // finished, or failed: wait reports which
@@ -1018,6 +1086,13 @@ int spoa_hip_share_stream(spoa_hip_engine* E, spoa_hip_engine* owner) {
   }
   if (E->pending.active) (void)RunPassFinish(E);
   E->compute = E->copy = owner ? owner->stream : E->stream;
+  return SPOA_HIP_OK;
+}
+
+int spoa_hip_set_graph_mode(spoa_hip_engine* E, int32_t on) {
+  if (!E) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null engine");
+  if (E->pending.active) (void)RunPassFinish(E);
+  E->graph_mode = on != 0;
   return SPOA_HIP_OK;
 }
 

@@ -123,3 +123,46 @@ def test_context_reuse_and_budget_split():
     res = ctx.run(ws)
     assert res.consensus == ref.consensus
     assert res.msa == ref.msa
+
+
+def test_autotuner_picks_configs_and_keeps_results(tmp_path):
+    """The autotuner times launch configurations per score width, installs
+    the fastest, caches it, and the batched results do not change."""
+    c = spoa_amd.CONFIGS[2]
+    ws = spoa_amd.config_windows(2, seed=5, n_windows=16)
+    ref = spoa_amd.run_windows(ws, *c["params"], mode="batched", msa=True)
+    ctx = spoa_amd.BatchContext(*c["params"], msa=True)
+    cache = str(tmp_path / "tune.json")
+    tuned = ctx.autotune(ws, rounds=3, cache=cache)
+    widths = {t["width"]: t for t in tuned}
+    assert set(widths) == {16, 32}
+    t16 = widths[16]
+    assert t16["cols_per_lane"] in (16, 32) and 1 <= t16["waves"] <= 8  # convex int16 lanes hold 16+
+    assert t16["kernel_ms"] > 0 and not t16["cached"]
+    res = ctx.run(ws)
+    assert res.consensus == ref.consensus
+    assert res.msa == ref.msa
+    # a second context reads the cache instead of measuring
+    ctx2 = spoa_amd.BatchContext(*c["params"], msa=True)
+    tuned2 = ctx2.autotune(ws, rounds=3, cache=cache)
+    t2 = {t["width"]: t for t in tuned2}[16]
+    assert t2["cached"] and (t2["cols_per_lane"], t2["waves"]) == (t16["cols_per_lane"], t16["waves"])
+    res2 = ctx2.run(ws)
+    assert res2.consensus == ref.consensus
+
+
+def test_graph_mode_replays_match_stream_mode():
+    """Every pass launched as a captured HIP graph: the same results as the
+    stream launches, run after run (each run replays new graphs), with
+    eager stream-mode runs in between."""
+    c = spoa_amd.CONFIGS[2]
+    ws = spoa_amd.config_windows(2, seed=8, n_windows=12)
+    eager = spoa_amd.BatchContext(*c["params"], msa=True)
+    graph = spoa_amd.BatchContext(*c["params"], msa=True, graph=True)
+    ref = eager.run(ws)
+    for _ in range(3):
+        res = graph.run(ws)
+        assert res.consensus == ref.consensus
+        assert res.msa == ref.msa
+        again = eager.run(ws)
+        assert again.consensus == ref.consensus
