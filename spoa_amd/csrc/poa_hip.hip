@@ -467,7 +467,9 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
       uint64_t rest = 0;
       for (size_t y = x; y < nj; ++y) rest += plans[lo + y].mat_bytes;
       const uint64_t room = E->budget > E->arena_bytes ? E->budget - E->arena_bytes : 0;
-      uint64_t want = std::min(room, std::max<uint64_t>(rest, 4ull << 30));
+      // grow geometrically (at least doubling the arena) so that batches
+      // whose jobs grow round by round allocate O(log) times
+      uint64_t want = std::min(room, std::max<uint64_t>(std::max<uint64_t>(rest, 4ull << 30), E->arena_bytes));
       if (want < need) {
         // fragmented: start over with one segment holding the whole pass
         if (retried) return Fail(SPOA_HIP_NO_MEMORY, kMsgNoMemory);
