@@ -60,64 +60,6 @@ std::string ReverseComplement(const char* s, std::size_t len) {
   return out;
 }
 
-// ------------------------------------------------------------ ThreadPool --
-
-ThreadPool::ThreadPool(unsigned threads) {
-  if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
-  for (unsigned t = 1; t < threads; ++t) workers_.emplace_back([this] { Work(); });
-}
-
-ThreadPool::~ThreadPool() {
-  {
-    std::lock_guard<std::mutex> lk(mu_);
-    stop_ = true;
-  }
-  cv_.notify_all();
-  for (auto& t : workers_) t.join();
-}
-
-void ThreadPool::Work() {
-  std::uint64_t seen = 0;
-  std::unique_lock<std::mutex> lk(mu_);
-  while (true) {
-    cv_.wait(lk, [&] { return stop_ || epoch_ != seen; });
-    if (stop_) return;
-    seen = epoch_;
-    ++active_;
-    // items are taken only while the loop that published them is current
-    while (epoch_ == seen && next_ < n_) {
-      const std::size_t i = next_++;
-      const std::function<void(std::size_t)>* fn = fn_;
-      lk.unlock();
-      (*fn)(i);
-      lk.lock();
-      ++finished_;
-    }
-    --active_;
-    done_cv_.notify_all();
-  }
-}
-
-void ThreadPool::For(std::size_t n, const std::function<void(std::size_t)>& fn) {
-  if (n == 0) return;
-  std::unique_lock<std::mutex> lk(mu_);
-  fn_ = &fn;
-  n_ = n;
-  next_ = 0;
-  finished_ = 0;
-  ++epoch_;
-  cv_.notify_all();
-  while (next_ < n_) {
-    const std::size_t i = next_++;
-    lk.unlock();
-    fn(i);
-    lk.lock();
-    ++finished_;
-  }
-  done_cv_.wait(lk, [&] { return finished_ == n_ && active_ == 0; });
-  fn_ = nullptr;
-}
-
 // --------------------------------------------------------------- drivers --
 
 BatchContext::BatchContext(const BatchOptions& opt) : opt_(opt), pool_(opt.threads) {
@@ -138,6 +80,12 @@ BatchContext::BatchContext(const BatchOptions& opt) : opt_(opt), pool_(opt.threa
   if (opt.graph) {
     for (auto* d : dev) spoa_hip_set_graph_mode(d, 1);
   }
+}
+
+std::vector<PoaGraph>& BatchContext::graphs(std::size_t n) {
+  if (graphs_.size() < n) graphs_.resize(n);
+  pool_.For(n, [&](std::size_t w) { graphs_[w].Clear(); });
+  return graphs_;
 }
 
 std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats) {
@@ -168,7 +116,7 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
   const BatchOptions& opt = ctx.options();
   ThreadPool& pool = ctx.pool();
   const std::uint32_t W = ws.n_windows;
-  std::vector<PoaGraph> graphs(W);
+  std::vector<PoaGraph>& graphs = ctx.graphs(W);
   std::vector<WindowResult> out(W);
   std::vector<std::string> rc_seq(W);
   const int per_read = opt.strand ? 2 : 1;

@@ -17,31 +17,12 @@
 #include <thread>
 #include <vector>
 
+#include "poa_graph.hpp"
+#include "thread_pool.hpp"
 #include "spoa/spoa.hpp"
 #include "spoa_hip.h"
 
 namespace spoa_amd {
-
-// A fixed pool of host threads running index-parallel loops.
-class ThreadPool {
- public:
-  explicit ThreadPool(unsigned threads);
-  ~ThreadPool();
-  unsigned size() const { return static_cast<unsigned>(workers_.size()) + 1; }
-  // runs fn(i) for i in [0, n), the calling thread included
-  void For(std::size_t n, const std::function<void(std::size_t)>& fn);
-
- private:
-  void Work();
-  std::vector<std::thread> workers_;
-  std::mutex mu_;
-  std::condition_variable cv_, done_cv_;
-  const std::function<void(std::size_t)>* fn_ = nullptr;
-  std::size_t n_ = 0, next_ = 0, finished_ = 0;
-  std::uint64_t epoch_ = 0;
-  unsigned active_ = 0;
-  bool stop_ = false;
-};
 
 // Flattened windows: window w holds reads [win_read_off[w], win_read_off[w+1]);
 // read k is seqs[read_off[k], read_off[k+1]) (quals likewise, or null).
@@ -98,11 +79,17 @@ class BatchContext {
   spoa::AlignmentEngine* engine(int half = 0) { return engines_[half].get(); }
   BatchOptions& mutable_options() { return opt_; }
   ThreadPool& pool() { return pool_; }
+  // The window graphs of a run: at least n, each cleared.  They keep their
+  // storage from run to run (the context holds the largest batch's graph
+  // memory until it is destroyed), so a steady stream of batches allocates
+  // and frees no graph memory.
+  std::vector<PoaGraph>& graphs(std::size_t n);
 
  private:
   BatchOptions opt_;
   std::unique_ptr<spoa::AlignmentEngine> engines_[2];
   ThreadPool pool_;
+  std::vector<PoaGraph> graphs_;
 };
 
 // Batched form.  Throws std::invalid_argument with the engine's message on

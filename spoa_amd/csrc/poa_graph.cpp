@@ -14,38 +14,38 @@ void PoaGraph::Clear() {
   for (int i = 0; i < 256; ++i) coder_[i] = decoder_[i] = -1;
   num_codes_ = 0;
   code_.clear();
-  first_in_.clear();
-  last_in_.clear();
-  first_out_.clear();
-  last_out_.clear();
+  in_.clear();
+  out_.clear();
   aligned_.clear();
   tail_.clear();
   head_.clear();
-  next_in_.clear();
-  next_out_.clear();
   weight_.clear();
-  paths_.clear();
+  path_.clear();
+  path_off_.assign(1, 0);
   rank_to_node_.clear();
   consensus_.clear();
-  RebuildView();
+  rank_of_.clear();
+  v_char_.clear();
+  v_sink_.clear();
+  v_id_.clear();
+  v_pred_.clear();
+  v_off_.assign(1, 0);
 }
 
 // src/graph.cpp:76-79
 std::uint32_t PoaGraph::AddNode(std::uint32_t code) {
   code_.push_back(static_cast<std::uint8_t>(code));
-  first_in_.push_back(kNil);
-  last_in_.push_back(kNil);
-  first_out_.push_back(kNil);
-  last_out_.push_back(kNil);
-  aligned_.emplace_back();
+  in_.add_owner();
+  out_.add_owner();
+  aligned_.add_owner();
   return num_nodes() - 1;
 }
 
 // src/graph.cpp:81-91: a tail->head edge that exists gains the weight
 void PoaGraph::AddEdge(std::uint32_t t, std::uint32_t h, std::uint32_t w) {
-  for (std::uint32_t e = first_out_[t]; e != kNil; e = next_out_[e]) {
-    if (head_[e] == h) {
-      weight_[e] += w;
+  for (const std::uint32_t* e = out_.begin(t); e != out_.end(t); ++e) {
+    if (head_[*e] == h) {
+      weight_[*e] += w;
       return;
     }
   }
@@ -53,34 +53,20 @@ void PoaGraph::AddEdge(std::uint32_t t, std::uint32_t h, std::uint32_t w) {
   tail_.push_back(t);
   head_.push_back(h);
   weight_.push_back(w);
-  next_in_.push_back(kNil);
-  next_out_.push_back(kNil);
-  if (last_out_[t] == kNil) {
-    first_out_[t] = e;
-  } else {
-    next_out_[last_out_[t]] = e;
-  }
-  last_out_[t] = e;
-  if (last_in_[h] == kNil) {
-    first_in_[h] = e;
-  } else {
-    next_in_[last_in_[h]] = e;
-  }
-  last_in_[h] = e;
+  out_.push(t, e);
+  in_.push(h, e);
 }
 
-// src/graph.cpp:93-110: a fresh chain for seq[begin, end); returns its
-// first node or -1
+// src/graph.cpp:93-110: a fresh chain for seq[begin, end) (consecutive node
+// ids); returns its first node or -1
 std::int64_t PoaGraph::AddSequence(const char* seq, const std::uint32_t* w, std::uint32_t begin, std::uint32_t end,
-                                   std::vector<std::uint32_t>* path) {
+                                   bool on_path) {
   if (begin == end) return -1;
-  std::int64_t first = -1, prev = -1;
+  const std::int64_t first = num_nodes();
   for (std::uint32_t i = begin; i < end; ++i) {
     const std::uint32_t curr = AddNode(static_cast<std::uint32_t>(coder_[static_cast<std::uint8_t>(seq[i])]));
-    if (prev >= 0) AddEdge(static_cast<std::uint32_t>(prev), curr, w[i - 1] + w[i]);
-    if (first < 0) first = curr;
-    prev = curr;
-    path->push_back(curr);
+    if (i > begin) AddEdge(curr - 1, curr, w[i - 1] + w[i]);
+    if (on_path) path_.push_back(curr);
   }
   return first;
 }
@@ -96,10 +82,9 @@ void PoaGraph::AddAlignment(const std::int32_t* pairs, std::uint32_t n_pairs, co
       decoder_[num_codes_++] = static_cast<std::uint8_t>(seq[i]);
     }
   }
-  std::vector<std::uint32_t> path;
   if (n_pairs == 0) {
-    AddSequence(seq, w, 0, len, &path);
-    paths_.push_back(std::move(path));
+    AddSequence(seq, w, 0, len, true);
+    path_off_.push_back(static_cast<std::uint32_t>(path_.size()));
     TopologicalSort();
     return;
   }
@@ -118,10 +103,10 @@ void PoaGraph::AddAlignment(const std::int32_t* pairs, std::uint32_t n_pairs, co
   }
   // unaligned prefix and suffix become fresh chains (the suffix is created
   // before the aligned nodes but visited after them)
-  const std::int64_t begin0 = AddSequence(seq, w, 0, static_cast<std::uint32_t>(first_pos), &path);
+  const std::int64_t begin0 = AddSequence(seq, w, 0, static_cast<std::uint32_t>(first_pos), true);
   std::int64_t prev = begin0 >= 0 ? static_cast<std::int64_t>(num_nodes()) - 1 : -1;
-  std::vector<std::uint32_t> suffix;
-  const std::int64_t last = AddSequence(seq, w, static_cast<std::uint32_t>(last_pos) + 1, len, &suffix);
+  const std::uint32_t suffix_len = len - static_cast<std::uint32_t>(last_pos) - 1;
+  const std::int64_t last = AddSequence(seq, w, static_cast<std::uint32_t>(last_pos) + 1, len, false);
   for (std::uint32_t k = 0; k < n_pairs; ++k) {
     const std::int32_t id = pairs[2 * k], pos = pairs[2 * k + 1];
     if (pos == -1) continue;
@@ -135,120 +120,114 @@ void PoaGraph::AddAlignment(const std::int32_t* pairs, std::uint32_t n_pairs, co
         curr = jt;
       } else {
         curr = kNil;
-        for (std::uint32_t kt : aligned_[jt]) {
-          if (code_[kt] == c) {
-            curr = kt;
+        for (const std::uint32_t* a = aligned_.begin(jt); a != aligned_.end(jt); ++a) {
+          if (code_[*a] == c) {
+            curr = *a;
             break;
           }
         }
         if (curr == kNil) {
           curr = AddNode(c);
-          const std::vector<std::uint32_t> group = aligned_[jt];
-          for (std::uint32_t kt : group) {
-            aligned_[kt].push_back(curr);
-            aligned_[curr].push_back(kt);
+          // jt's list is read by index: pushes to other lists may move the pool
+          const std::uint32_t group = aligned_.size(jt);
+          for (std::uint32_t i = 0; i < group; ++i) {
+            const std::uint32_t kt = aligned_.at(jt, i);
+            aligned_.push(kt, curr);
+            aligned_.push(curr, kt);
           }
-          aligned_[jt].push_back(curr);
-          aligned_[curr].push_back(jt);
+          aligned_.push(jt, curr);
+          aligned_.push(curr, jt);
         }
       }
     }
     if (prev >= 0) AddEdge(static_cast<std::uint32_t>(prev), curr, w[pos - 1] + w[pos]);
     prev = curr;
-    path.push_back(curr);
+    path_.push_back(curr);
   }
   if (last >= 0) {
     AddEdge(static_cast<std::uint32_t>(prev), static_cast<std::uint32_t>(last), w[last_pos] + w[last_pos + 1]);
+    for (std::uint32_t i = 0; i < suffix_len; ++i) path_.push_back(static_cast<std::uint32_t>(last) + i);
   }
-  path.insert(path.end(), suffix.begin(), suffix.end());
-  paths_.push_back(std::move(path));
+  path_off_.push_back(static_cast<std::uint32_t>(path_.size()));
   TopologicalSort();
 }
 
 // src/graph.cpp:249-303: depth-first over node ids; a node is finished once
 // its in-edge tails (and, for a group's first member, its aligned nodes)
-// are; the first member is ranked with its whole group.
+// are; the first member is ranked with its whole group.  The kernel view
+// is written as nodes are ranked (every predecessor of a node is ranked
+// before it).
 void PoaGraph::TopologicalSort() {
   const std::uint32_t V = num_nodes();
-  rank_to_node_.clear();
+  // per node: 1 = visited, 2 = finished, 4 = ranked with its group's first member
   marks_.assign(V, 0);
-  ignored_.assign(V, 0);
-  stack_.clear();
-  // the kernel view is built as nodes are ranked: every predecessor of a
-  // node is ranked before it (the order is topological)
+  rank_to_node_.resize(V);
   rank_of_.resize(V);
-  v_char_.clear();
-  v_sink_.clear();
-  v_id_.clear();
-  v_off_.clear();
-  v_pred_.clear();
-  auto emit = [&](std::uint32_t id) {
-    rank_of_[id] = static_cast<std::uint32_t>(rank_to_node_.size());
-    rank_to_node_.push_back(id);
-    v_off_.push_back(static_cast<std::uint32_t>(v_pred_.size()));
-    for (std::uint32_t e = first_in_[id]; e != kNil; e = next_in_[e]) v_pred_.push_back(rank_of_[tail_[e]]);
-    v_char_.push_back(static_cast<std::uint8_t>(decoder_[code_[id]]));
-    v_id_.push_back(static_cast<std::int32_t>(id));
-    v_sink_.push_back(first_out_[id] == kNil ? 1 : 0);
-  };
-  for (std::uint32_t root = 0; root < V; ++root) {
-    if (marks_[root] != 0) continue;
-    stack_.push_back(root);
-    while (!stack_.empty()) {
-      const std::uint32_t curr = stack_.back();
-      bool valid = true;
-      if (marks_[curr] != 2) {
-        for (std::uint32_t e = first_in_[curr]; e != kNil; e = next_in_[e]) {
-          const std::uint32_t t = tail_[e];
-          if (marks_[t] != 2) {
-            stack_.push_back(t);
-            valid = false;
-          }
-        }
-        if (!ignored_[curr]) {
-          for (std::uint32_t a : aligned_[curr]) {
-            if (marks_[a] != 2) {
-              stack_.push_back(a);
-              ignored_[a] = 1;
-              valid = false;
-            }
-          }
-        }
-        if (valid) {
-          marks_[curr] = 2;
-          if (!ignored_[curr]) {
-            emit(curr);
-            for (std::uint32_t a : aligned_[curr]) emit(a);
-          }
-        } else {
-          marks_[curr] = 1;
-        }
-      }
-      if (valid) stack_.pop_back();
-    }
-  }
-  v_off_.push_back(static_cast<std::uint32_t>(v_pred_.size()));
-}
-
-void PoaGraph::RebuildView() {
-  const std::uint32_t V = static_cast<std::uint32_t>(rank_to_node_.size());
-  rank_of_.resize(num_nodes());
-  for (std::uint32_t r = 0; r < V; ++r) rank_of_[rank_to_node_[r]] = r;
   v_char_.resize(V);
   v_sink_.resize(V);
   v_id_.resize(V);
   v_off_.resize(V + 1);
-  v_pred_.clear();
-  v_pred_.reserve(tail_.size());
-  for (std::uint32_t r = 0; r < V; ++r) {
-    const std::uint32_t id = rank_to_node_[r];
-    v_off_[r] = static_cast<std::uint32_t>(v_pred_.size());
-    for (std::uint32_t e = first_in_[id]; e != kNil; e = next_in_[e]) v_pred_.push_back(rank_of_[tail_[e]]);
-    v_char_[r] = static_cast<std::uint8_t>(decoder_[code_[id]]);
-    v_id_[r] = static_cast<std::int32_t>(id);
-    v_sink_[r] = first_out_[id] == kNil ? 1 : 0;
+  v_pred_.resize(tail_.size());
+  std::uint32_t* const r2n = rank_to_node_.data();
+  std::uint32_t* const rof = rank_of_.data();
+  std::uint32_t* const off = v_off_.data();
+  std::uint32_t* const pred = v_pred_.data();
+  std::uint8_t* const vch = v_char_.data();
+  std::uint8_t* const vsk = v_sink_.data();
+  std::int32_t* const vid = v_id_.data();
+  std::uint8_t* const mk = marks_.data();
+  const std::uint32_t* const tl = tail_.data();
+  const std::uint8_t* const cd = code_.data();
+  std::uint32_t r = 0, p = 0;
+  auto emit = [&](std::uint32_t id) {
+    rof[id] = r;
+    r2n[r] = id;
+    off[r] = p;
+    for (const std::uint32_t* e = in_.begin(id); e != in_.end(id); ++e) pred[p++] = rof[tl[*e]];
+    vch[r] = static_cast<std::uint8_t>(decoder_[cd[id]]);
+    vid[r] = static_cast<std::int32_t>(id);
+    vsk[r] = out_.size(id) == 0 ? 1 : 0;
+    ++r;
+  };
+  stack_.clear();
+  for (std::uint32_t root = 0; root < V; ++root) {
+    if (mk[root] != 0) continue;
+    stack_.push_back(root);
+    while (!stack_.empty()) {
+      const std::uint32_t curr = stack_.back();
+      if ((mk[curr] & 2) == 0) {
+        bool valid = true;
+        for (const std::uint32_t* e = in_.begin(curr); e != in_.end(curr); ++e) {
+          const std::uint32_t t = tl[*e];
+          if ((mk[t] & 2) == 0) {
+            stack_.push_back(t);
+            valid = false;
+          }
+        }
+        const bool ignored = (mk[curr] & 4) != 0;
+        if (!ignored) {
+          for (const std::uint32_t* a = aligned_.begin(curr); a != aligned_.end(curr); ++a) {
+            if ((mk[*a] & 2) == 0) {
+              stack_.push_back(*a);
+              mk[*a] |= 4;
+              valid = false;
+            }
+          }
+        }
+        if (!valid) {
+          mk[curr] |= 1;
+          continue;
+        }
+        mk[curr] |= 2;
+        if (!ignored) {
+          emit(curr);
+          for (const std::uint32_t* a = aligned_.begin(curr); a != aligned_.end(curr); ++a) emit(*a);
+        }
+      }
+      stack_.pop_back();
+    }
   }
-  v_off_[V] = static_cast<std::uint32_t>(v_pred_.size());
+  off[V] = p;
 }
 
 spoa_hip_graph PoaGraph::View() const {
@@ -266,9 +245,10 @@ spoa_hip_graph PoaGraph::View() const {
 std::uint32_t PoaGraph::BranchCompletion(std::uint32_t rank, std::vector<std::int64_t>& score,
                                          std::vector<std::uint32_t>& pred) {
   const std::uint32_t start = rank_to_node_[rank];
-  for (std::uint32_t e = first_out_[start]; e != kNil; e = next_out_[e]) {
-    for (std::uint32_t f = first_in_[head_[e]]; f != kNil; f = next_in_[f]) {
-      if (tail_[f] != start) score[tail_[f]] = -1;
+  for (const std::uint32_t* e = out_.begin(start); e != out_.end(start); ++e) {
+    const std::uint32_t hd = head_[*e];
+    for (const std::uint32_t* f = in_.begin(hd); f != in_.end(hd); ++f) {
+      if (tail_[*f] != start) score[tail_[*f]] = -1;
     }
   }
   std::uint32_t best = kNil;
@@ -276,8 +256,8 @@ std::uint32_t PoaGraph::BranchCompletion(std::uint32_t rank, std::vector<std::in
     const std::uint32_t it = rank_to_node_[i];
     score[it] = -1;
     pred[it] = kNil;
-    for (std::uint32_t e = first_in_[it]; e != kNil; e = next_in_[e]) {
-      const std::uint32_t t = tail_[e];
+    for (const std::uint32_t* ep = in_.begin(it); ep != in_.end(it); ++ep) {
+      const std::uint32_t e = *ep, t = tail_[e];
       if (score[t] == -1) continue;
       if (score[it] < weight_[e] || (score[it] == weight_[e] && score[pred[it]] <= score[t])) {
         score[it] = weight_[e];
@@ -297,8 +277,8 @@ void PoaGraph::TraverseHeaviestBundle() {
   std::vector<std::int64_t> score(num_nodes(), -1);
   std::uint32_t best = kNil;
   for (std::uint32_t it : rank_to_node_) {
-    for (std::uint32_t e = first_in_[it]; e != kNil; e = next_in_[e]) {
-      const std::uint32_t t = tail_[e];
+    for (const std::uint32_t* ep = in_.begin(it); ep != in_.end(it); ++ep) {
+      const std::uint32_t e = *ep, t = tail_[e];
       if (score[it] < weight_[e] || (score[it] == weight_[e] && score[pred[it]] <= score[t])) {
         score[it] = weight_[e];
         pred[it] = t;
@@ -307,7 +287,7 @@ void PoaGraph::TraverseHeaviestBundle() {
     if (pred[it] != kNil) score[it] += score[pred[it]];
     if (best == kNil || score[best] < score[it]) best = it;
   }
-  while (first_out_[best] != kNil) best = BranchCompletion(rank_of_[best], score, pred);
+  while (out_.size(best) != 0) best = BranchCompletion(rank_of_[best], score, pred);
   consensus_.clear();
   for (std::uint32_t v = best; v != kNil; v = pred[v]) consensus_.push_back(v);
   std::reverse(consensus_.begin(), consensus_.end());
@@ -330,8 +310,8 @@ std::vector<std::uint32_t> PoaGraph::MsaColumns(std::uint32_t* width) const {
   for (std::uint32_t i = 0; i < rank_to_node_.size(); ++i, ++j) {
     const std::uint32_t it = rank_to_node_[i];
     col[it] = j;
-    for (std::uint32_t a : aligned_[it]) {
-      col[a] = j;
+    for (const std::uint32_t* a = aligned_.begin(it); a != aligned_.end(it); ++a) {
+      col[*a] = j;
       ++i;
     }
   }
@@ -344,10 +324,13 @@ std::vector<std::string> PoaGraph::GenerateMultipleSequenceAlignment(bool includ
   std::uint32_t width = 0;
   const std::vector<std::uint32_t> col = MsaColumns(&width);
   std::vector<std::string> rows;
-  rows.reserve(paths_.size() + (include_consensus ? 1 : 0));
-  for (const auto& path : paths_) {
+  rows.reserve(num_sequences() + (include_consensus ? 1 : 0));
+  for (std::uint32_t s = 0; s < num_sequences(); ++s) {
     std::string row(width, '-');
-    for (std::uint32_t v : path) row[col[v]] = static_cast<char>(decoder_[code_[v]]);
+    for (std::uint32_t k = path_off_[s]; k < path_off_[s + 1]; ++k) {
+      const std::uint32_t v = path_[k];
+      row[col[v]] = static_cast<char>(decoder_[code_[v]]);
+    }
     rows.push_back(std::move(row));
   }
   if (include_consensus) {

@@ -4,12 +4,15 @@
 // spoa::Graph (include/spoa/graph.hpp) keeps the reference's public shape:
 // heap nodes and edges linked by pointers.  The batched form merges
 // thousands of windows per round and only needs what the alignment and
-// the outputs read, so this graph stores the same information in flat
-// arrays: node codes, edges with per-head and per-tail lists threaded in
-// creation order (the order of Node::inedges / Node::outedges), aligned
-// groups, and each sequence's path instead of per-edge labels (a read
-// visits a node at most once, so following its label from its first node,
-// src/graph.cpp:22-30 and 339-366, walks exactly its path).
+// the outputs read, so this graph stores the same information in a fixed
+// set of flat arrays: node codes; edges; per node its in-edges, out-edges
+// and aligned nodes as lists in creation order (the order of
+// Node::inedges / Node::outedges / Node::aligned_nodes), each list a block
+// of one shared pool; and each sequence's path instead of per-edge labels
+// (a read visits a node at most once, so following its label from its
+// first node, src/graph.cpp:22-30 and 339-366, walks exactly its path).
+// Clear() keeps every array's capacity, so a graph reused for the next
+// window allocates nothing once it has seen a window of that size.
 //
 // Every operation restates the reference function it replaces (cited per
 // function) and is checked against the oracle in tests/test_host_graph.py.
@@ -23,6 +26,46 @@
 
 namespace spoa_amd {
 
+// Per-owner lists of uint32 in one pool: a list is a block [start, start +
+// size) of capacity cap; a full list moves to a block of twice the capacity
+// at the end of the pool (or grows in place when it is the last block).
+class BlockLists {
+ public:
+  void clear() {
+    pool_.clear();
+    meta_.clear();
+  }
+  void add_owner() { meta_.push_back(Meta{0, 0, 0}); }
+  std::uint32_t size(std::uint32_t o) const { return meta_[o].size; }
+  const std::uint32_t* begin(std::uint32_t o) const { return pool_.data() + meta_[o].start; }
+  const std::uint32_t* end(std::uint32_t o) const { return pool_.data() + meta_[o].start + meta_[o].size; }
+  std::uint32_t at(std::uint32_t o, std::uint32_t i) const { return pool_[meta_[o].start + i]; }
+  void push(std::uint32_t o, std::uint32_t v) {
+    Meta& m = meta_[o];
+    if (m.size == m.cap) Grow(m);
+    pool_[m.start + m.size++] = v;
+  }
+
+ private:
+  struct Meta {
+    std::uint32_t start, size, cap;
+  };
+  void Grow(Meta& m) {
+    const std::uint32_t cap = m.cap ? 2 * m.cap : 2;
+    const std::size_t end = pool_.size();
+    if (m.cap != 0 && m.start + m.cap == end) {
+      pool_.resize(end + (cap - m.cap));
+    } else {
+      pool_.resize(end + cap);
+      for (std::uint32_t i = 0; i < m.size; ++i) pool_[end + i] = pool_[m.start + i];
+      m.start = static_cast<std::uint32_t>(end);
+    }
+    m.cap = cap;
+  }
+  std::vector<std::uint32_t> pool_;
+  std::vector<Meta> meta_;
+};
+
 class PoaGraph {
  public:
   PoaGraph();
@@ -30,6 +73,7 @@ class PoaGraph {
   void Clear();
   std::uint32_t num_nodes() const { return static_cast<std::uint32_t>(code_.size()); }
   std::uint32_t num_codes() const { return num_codes_; }
+  std::uint32_t num_sequences() const { return static_cast<std::uint32_t>(path_off_.size() - 1); }
 
   // Graph::AddAlignment (src/graph.cpp:155-247): pairs (node id or -1,
   // position or -1) in path order; weights[i] per base.
@@ -53,9 +97,8 @@ class PoaGraph {
   std::uint32_t AddNode(std::uint32_t code);
   void AddEdge(std::uint32_t tail, std::uint32_t head, std::uint32_t weight);
   std::int64_t AddSequence(const char* seq, const std::uint32_t* w, std::uint32_t begin, std::uint32_t end,
-                           std::vector<std::uint32_t>* path);
+                           bool on_path);
   void TopologicalSort();
-  void RebuildView();
   void TraverseHeaviestBundle();
   std::uint32_t BranchCompletion(std::uint32_t rank, std::vector<std::int64_t>& score,
                                  std::vector<std::uint32_t>& pred);
@@ -65,15 +108,14 @@ class PoaGraph {
   std::int32_t decoder_[256];
   std::uint32_t num_codes_ = 0;
 
-  // nodes
+  // nodes: code, in-edges, out-edges (edge ids), aligned nodes (node ids)
   std::vector<std::uint8_t> code_;
-  std::vector<std::uint32_t> first_in_, last_in_, first_out_, last_out_;
-  std::vector<std::vector<std::uint32_t>> aligned_;
-  // edges, threaded per head / per tail in creation order
-  std::vector<std::uint32_t> tail_, head_, next_in_, next_out_;
+  BlockLists in_, out_, aligned_;
+  // edges
+  std::vector<std::uint32_t> tail_, head_;
   std::vector<std::int64_t> weight_;
-  // sequences: the nodes each one visits, in order
-  std::vector<std::vector<std::uint32_t>> paths_;
+  // sequences: sequence s visits path_[path_off_[s] .. path_off_[s + 1])
+  std::vector<std::uint32_t> path_, path_off_;
 
   std::vector<std::uint32_t> rank_to_node_;
   std::vector<std::uint32_t> consensus_;
@@ -84,7 +126,7 @@ class PoaGraph {
   std::vector<std::uint32_t> v_off_, v_pred_, rank_of_;
 
   // scratch of the topological sort
-  std::vector<std::uint8_t> marks_, ignored_;
+  std::vector<std::uint8_t> marks_;
   std::vector<std::uint32_t> stack_;
 };
 

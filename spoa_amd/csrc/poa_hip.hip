@@ -17,6 +17,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -24,6 +25,7 @@
 
 #include "poa_kernels.hpp"
 #include "spoa_hip.h"
+#include "thread_pool.hpp"
 
 using spoa_hip_dev::DevJob;
 using spoa_hip_dev::KArgs;
@@ -269,6 +271,8 @@ struct Plan {
 
 struct spoa_hip_engine {
   int device = 0;
+  // host threads packing and unpacking large passes (made on first use)
+  std::unique_ptr<spoa_amd::ThreadPool> pool;
   hipStream_t stream = nullptr;   // the engine's own stream
   hipStream_t copy = nullptr;     // copies: the own stream, or a shared one
   // kernels: the engine's own stream, or one shared with other engines so
@@ -399,6 +403,19 @@ void PackJob(const spoa_hip_params& P, const spoa_hip_job& j, const Plan& pl, ui
   std::memset(s + j.len, 0, pl.seq_bytes - j.len);
 }
 
+// fn(a, b) over [0, n) in chunks: on the engine's host threads for large
+// passes, on the calling thread for small ones.
+template <typename Fn>
+void ParallelRanges(spoa_hip_engine* E, size_t n, const Fn& fn) {
+  constexpr size_t kChunk = 16;
+  if (n < 4 * kChunk) {
+    fn(size_t{0}, n);
+    return;
+  }
+  if (!E->pool) E->pool.reset(new spoa_amd::ThreadPool(std::max(1u, std::min(16u, std::thread::hardware_concurrency()))));
+  E->pool->For((n + kChunk - 1) / kChunk, [&](size_t c) { fn(c * kChunk, std::min(n, (c + 1) * kChunk)); });
+}
+
 KernelFn GroupKernel(spoa_hip_engine* E, const Plan& pl) {
   return PickKernel(E->prm.subtype, pl.width, pl.cfg.k, pl.cfg.waves);
 }
@@ -519,17 +536,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
               &d.nslots, &d.slot_chars);
     }
   };
-  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  const unsigned nt = nj >= 64 ? hw : 1u;
-  if (nt > 1) {
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; ++t) {
-      th.emplace_back(pack_range, static_cast<size_t>(nj) * t / nt, static_cast<size_t>(nj) * (t + 1) / nt);
-    }
-    for (auto& t : th) t.join();
-  } else {
-    pack_range(0, nj);
-  }
+  ParallelRanges(E, nj, pack_range);
   E->stats.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
 
   // ---- launch groups
@@ -724,34 +731,41 @@ int RunPassFinish(spoa_hip_engine* E) {
   // ---- unpack: pairs come back in traceback order
   const int32_t* res = E->h_out.At<int32_t>(pd.off_res);
   const int32_t* allp = E->h_out.At<int32_t>(pd.off_pairs);
-  int first = SPOA_HIP_OK;
-  for (size_t x = lo; x < hi; ++x) {
-    spoa_hip_job& j = jobs[plans[x].job];
-    const int32_t* r4 = res + 4 * (x - lo);
-    j.width_bits = plans[x].width;
-    j.has_score = r4[2];
-    j.score = r4[1];
-    const uint32_t np = static_cast<uint32_t>(r4[0]);
-    int status = r4[3] == spoa_hip_dev::kDevCapacity
-                     ? SPOA_HIP_CAPACITY
-                     : (r4[3] == spoa_hip_dev::kDevInconsistent ? SPOA_HIP_INCONSISTENT : SPOA_HIP_OK);
-    if (status == SPOA_HIP_OK && np > j.cap) status = SPOA_HIP_CAPACITY;
-    if (status == SPOA_HIP_OK) {
-      const int32_t* src = allp + 2ull * pd.pair_at[x - lo];
-      for (uint32_t k = 0; k < np; ++k) {
-        j.pairs[2 * k] = src[2 * (np - 1 - k)];
-        j.pairs[2 * k + 1] = src[2 * (np - 1 - k) + 1];
+  auto unpack_range = [&](size_t a, size_t b) {
+    for (size_t x = lo + a; x < lo + b; ++x) {
+      spoa_hip_job& j = jobs[plans[x].job];
+      const int32_t* r4 = res + 4 * (x - lo);
+      j.width_bits = plans[x].width;
+      j.has_score = r4[2];
+      j.score = r4[1];
+      const uint32_t np = static_cast<uint32_t>(r4[0]);
+      int status = r4[3] == spoa_hip_dev::kDevCapacity
+                       ? SPOA_HIP_CAPACITY
+                       : (r4[3] == spoa_hip_dev::kDevInconsistent ? SPOA_HIP_INCONSISTENT : SPOA_HIP_OK);
+      if (status == SPOA_HIP_OK && np > j.cap) status = SPOA_HIP_CAPACITY;
+      if (status == SPOA_HIP_OK) {
+        const int32_t* src = allp + 2ull * pd.pair_at[x - lo];
+        for (uint32_t k = 0; k < np; ++k) {
+          j.pairs[2 * k] = src[2 * (np - 1 - k)];
+          j.pairs[2 * k + 1] = src[2 * (np - 1 - k) + 1];
+        }
+        j.n_pairs = np;
+      } else {
+        j.n_pairs = 0;
       }
-      j.n_pairs = np;
-    } else {
-      j.n_pairs = 0;
-      if (first == SPOA_HIP_OK) {
-        first = status;
-        g_last_error = status == SPOA_HIP_CAPACITY ? "[spoa::hip] alignment output capacity exceeded"
-                                                   : "[spoa::hip] traceback found no consistent move";
-      }
+      j.status = status;
     }
-    j.status = status;
+  };
+  ParallelRanges(E, hi - lo, unpack_range);
+  // the first failed job (in pass order) names the error
+  int first = SPOA_HIP_OK;
+  for (size_t x = lo; x < hi && first == SPOA_HIP_OK; ++x) {
+    const int status = jobs[plans[x].job].status;
+    if (status != SPOA_HIP_OK) {
+      first = status;
+      g_last_error = status == SPOA_HIP_CAPACITY ? "[spoa::hip] alignment output capacity exceeded"
+                                                 : "[spoa::hip] traceback found no consistent move";
+    }
   }
   E->stats.unpack_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wait1).count();
   return first;
