@@ -184,47 +184,213 @@ __device__ __forceinline__ void LoadChars(const uint8_t* __restrict__ seq, uint3
 }
 
 // ------------------------------------------------------------ traceback --
-// Serial walk of the reference traceback, run by one lane.  Pairs are
-// emitted in traceback order (the host reverses them).
+// The reference traceback, walked by one wave.  Every lane runs the same
+// serial walk (the same values, so control flow stays uniform); the lanes
+// together stage an LDS tile of what the walk reads next -- rows [r0, r1]
+// and columns [c0, c0 + C) of every stored matrix, the read's bases at those
+// columns, the predecessor offsets, node ids and bases of those rows and as
+// many of their predecessor lists as fit -- so that a step reads LDS instead of a chain of dependent device
+// loads.  The walk only moves to lower rows and columns; when it nears the
+// tile's lower or left edge it restages the tile below and left of its
+// position.  Reads outside the tile (long edges) go to device memory.
+// Pairs are emitted in traceback order (the host reverses them); lane 0
+// stores them.
+template <typename T>
+using Lds = __attribute__((address_space(3))) T;
+typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
+
+// Device-memory reads of the walk outside its tile (long edges), kept out
+// of line so that the tile reads stay LDS reads.
+template <typename T>
+__device__ __attribute__((noinline)) T TbLoad(const T* p) {
+  return *p;
+}
+
+template <typename ST>
+struct TbTile {
+  Lds<ST>* m;         // [NM][R][C]
+  Lds<uint32_t>* po;  // [R + 1] predecessor offsets of rows rb..r1 (and the end of r1)
+  Lds<U32x4>* rec;    // [R] per row: first predecessor (0 for a source), base, node id
+  Lds<uint32_t>* pr;  // [P] predecessors pb0 .. pb0 + pn
+  Lds<uint8_t>* sq;   // [C] read bases of the tile columns
+  int32_t R = 0, C = 0, P = 0, margin = 0;
+  bool on = false;
+  int32_t r0 = 1, r1 = 0, rb = 1, c0 = 0;
+  uint32_t pb0 = 0, pn = 0;
+};
+
 template <int SUB, typename ST>
 __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t* __restrict__ seq,
                           uint32_t n, const ST* __restrict__ mat, uint64_t rstride, uint32_t stride,
                           int32_t max_i, int32_t max_j, int32_t* __restrict__ out, uint32_t cap,
-                          uint32_t* n_out, int32_t* status_out) {
+                          uint32_t* n_out, int32_t* status_out, Lds<uint32_t>* lds, uint32_t lds_bytes, int lane) {
+  constexpr int NM = SUB == 0 ? 1 : (SUB == 1 ? 3 : 5);
+  constexpr int S = sizeof(ST);
+  constexpr int VEC = 16 / S;          // columns a 16-byte load
+  constexpr int TC = S == 2 ? 32 : 24;  // tile columns
   const int32_t g = prm.g, e = prm.e, q = prm.q, c = prm.c;
   const int type = prm.type;
+  auto U = [](int32_t v) -> int32_t { return static_cast<int32_t>(Uniform(static_cast<uint32_t>(v))); };
+
+  // tile geometry from the LDS the kernel lends (its query profile, dead now)
+  TbTile<ST> T;
+  {
+    // matrices, offset, record, ~3 predecessors
+    const uint32_t per_row = NM * TC * S + 4 + 16 + 3 * 4;
+    // 64 bytes spare: the end offset of the last row and the tile's bases
+    const int32_t R = static_cast<int32_t>((lds_bytes > 64 ? lds_bytes - 64 : 0) / per_row);
+    if (R >= 8 && stride >= static_cast<uint32_t>(TC)) {
+      T.R = R;
+      T.C = TC;
+      T.P = 3 * R;
+      T.margin = 4;
+      T.m = reinterpret_cast<Lds<ST>*>(lds);
+      T.rec = reinterpret_cast<Lds<U32x4>*>(lds + (NM * R * TC * S) / 4);
+      T.po = reinterpret_cast<Lds<uint32_t>*>(T.rec + R);
+      T.pr = T.po + R + 1;
+      T.sq = reinterpret_cast<Lds<uint8_t>*>(T.pr + T.P);
+      T.on = true;
+    }
+  }
+  // stage the tile whose top-right cell is (i, j + 1)
+  auto stage = [&](int32_t i, int32_t j) {
+    T.r1 = i;
+    T.r0 = max(0, i - T.R + 1);
+    int32_t c0 = j + 2 - T.C;
+    c0 = c0 <= 0 ? 0 : ((c0 + VEC - 1) / VEC) * VEC;
+    if (c0 + T.C > static_cast<int32_t>(stride)) c0 = static_cast<int32_t>(stride) - T.C;
+    T.c0 = c0;
+    const int32_t nrow = T.r1 - T.r0 + 1;
+    constexpr int32_t kPer = TC / VEC;  // 16-byte pieces of one row of one matrix
+    const int32_t pieces = nrow * NM * kPer;
+    for (int32_t x = lane; x < pieces; x += 64) {
+      const int32_t k = x % kPer, rt = x / kPer, t = rt % NM, r = rt / NM;
+      const U32x4 v = *reinterpret_cast<const U32x4*>(mat + static_cast<uint64_t>(T.r0 + r) * rstride +
+                                                      static_cast<uint64_t>(t) * stride +
+                                                      static_cast<uint32_t>(c0 + k * VEC));
+      *reinterpret_cast<Lds<U32x4>*>(T.m + (t * T.R + r) * TC + k * VEC) = v;
+    }
+    // rows rb..r1 carry graph data (row 0 has none)
+    T.rb = max(T.r0, 1);
+    const int32_t nm = T.r1 - T.rb + 1;
+    for (int32_t x = lane; x <= nm; x += 64) {
+      T.po[x] = gv.pred_off[T.rb - 1 + x];
+      if (x < nm) {
+        // the row record of the view holds the base and the first predecessor
+        const uint4 rw = gv.rows[T.rb - 1 + x];
+        T.rec[x] = U32x4{rw.y, rw.x & 0xffu, static_cast<uint32_t>(gv.node_id[T.rb - 1 + x]), 0u};
+      }
+    }
+    T.pb0 = nm > 0 ? Uniform(gv.pred_off[T.rb - 1]) : 0u;
+    const uint32_t tot = nm > 0 ? Uniform(gv.pred_off[T.r1]) - T.pb0 : 0u;
+    T.pn = min(tot, static_cast<uint32_t>(T.P));
+    for (uint32_t x = static_cast<uint32_t>(lane); x < T.pn; x += 64) T.pr[x] = gv.preds[T.pb0 + x];
+    if (lane < TC) T.sq[lane] = c0 + lane < static_cast<int32_t>(n) ? seq[c0 + lane] : 0;
+    // the wave's LDS writes land before any lane reads them
+    __asm__ volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+  };
+  auto ensure = [&](int32_t i, int32_t j) {
+    if (!T.on) return;
+    const bool inside = i <= T.r1 && (i - T.margin >= T.r0 || T.r0 == 0) && j + 1 < T.c0 + T.C &&
+                        (j - 1 >= T.c0 || T.c0 == 0);
+    if (!inside) stage(i, j);
+  };
+
+  // Every value the walk reads is the same in all lanes: it is moved to a
+  // scalar register, so the walk's arithmetic and branches are scalar.
+  // matrix t of row r at column col (t: 0 H, 1 F, 2 E, 3 O, 4 Q)
+  auto at = [&](int t, int32_t r, int32_t col) -> int32_t {
+    if (r >= T.r0 && r <= T.r1 && col >= T.c0 && col < T.c0 + T.C) {
+      return U(static_cast<int32_t>(T.m[(t * T.R + (r - T.r0)) * TC + (col - T.c0)]));
+    }
+    return U(static_cast<int32_t>(TbLoad(mat + static_cast<uint64_t>(r) * rstride + static_cast<uint64_t>(t) * stride +
+                                         static_cast<uint32_t>(col))));
+  };
+  // graph data of DP row r >= 1 (view index r - 1)
+  auto pred_begin = [&](int32_t r) -> uint32_t {
+    return Uniform(r >= T.rb && r <= T.r1 ? T.po[r - T.rb] : TbLoad(gv.pred_off + (r - 1)));
+  };
+  auto pred_end = [&](int32_t r) -> uint32_t {
+    return Uniform(r >= T.rb && r <= T.r1 ? T.po[r - T.rb + 1] : TbLoad(gv.pred_off + r));
+  };
+  auto pred_at = [&](uint32_t k) -> int32_t {
+    return U(static_cast<int32_t>(k - T.pb0 < T.pn ? T.pr[k - T.pb0] : TbLoad(gv.preds + k)));
+  };
+  auto node_of = [&](int32_t r) -> int32_t {
+    return U(r >= T.rb && r <= T.r1 ? static_cast<int32_t>(T.rec[r - T.rb].z) : TbLoad(gv.node_id + (r - 1)));
+  };
+  auto base_of = [&](int32_t r) -> uint32_t {
+    return Uniform(r >= T.rb && r <= T.r1 ? T.rec[r - T.rb].y : (TbLoad(&gv.rows[r - 1].x) & 0xffu));
+  };
+  auto prof = [&](int32_t r, int32_t col) -> int32_t {
+    if (col >= static_cast<int32_t>(n)) return prm.pad;
+    const uint32_t b = Uniform(col >= T.c0 && col < T.c0 + T.C ? T.sq[col - T.c0] : TbLoad(seq + col));
+    return b == base_of(r) ? prm.m : prm.mis;
+  };
+
   uint32_t np = 0;
   int32_t status = kDevOk;
+  // pair k waits in lane k % 64 until 64 pairs are gathered, then the wave
+  // stores them in one go
+  int32_t buf_a = 0, buf_b = 0;
   auto emit = [&](int32_t a, int32_t b) {
     if (np < cap) {
-      out[2 * np] = a;
-      out[2 * np + 1] = b;
+      const bool mine = lane == static_cast<int>(np & 63u);
+      buf_a = mine ? a : buf_a;
+      buf_b = mine ? b : buf_b;
+      if ((np & 63u) == 63u) {
+        const uint32_t k = np - 63u + static_cast<uint32_t>(lane);
+        *reinterpret_cast<int2*>(out + 2ull * k) = make_int2(buf_a, buf_b);
+      }
     } else {
       status = kDevCapacity;
     }
     ++np;
   };
-  // matrix t of row r at column col (t: 0 H, 1 F, 2 E, 3 O, 4 Q)
-  auto at = [&](int t, int32_t r, int32_t col) -> int32_t {
-    return static_cast<int32_t>(
-        mat[static_cast<uint64_t>(r) * rstride + static_cast<uint64_t>(t) * stride + static_cast<uint32_t>(col)]);
-  };
-  auto prof = [&](int32_t r, int32_t col) -> int32_t {
-    if (col >= static_cast<int32_t>(n)) return prm.pad;
-    const uint32_t ch = gv.rows[r - 1].x & 0xffu;
-    return seq[col] == ch ? prm.m : prm.mis;
-  };
-
-  int32_t i = max_i, j = max_j, prev_i = 0, prev_j = 0;
+  int32_t i = static_cast<int32_t>(Uniform(static_cast<uint32_t>(max_i)));
+  int32_t j = static_cast<int32_t>(Uniform(static_cast<uint32_t>(max_j)));
+  int32_t prev_i = 0, prev_j = 0;
   while (true) {
     // every step emits one pair, so a full buffer also bounds the walk
     if (j == -1 || i == 0 || status != kDevOk) break;
+    ensure(i, j);
+    // Diagonal run through first predecessors, all in the tile (the
+    // diagonal test below with k = 0 and j >= 1): each step reads the
+    // predecessor's cell, its record and the next base in one round.
+    if (T.on && i >= T.rb && i <= T.r1 && j >= T.c0 + 1 && j < T.c0 + T.C) {
+      bool moved = false;
+      int32_t h = at(0, i, j);
+      U32x4 rc = T.rec[i - T.rb];
+      int32_t p0 = U(static_cast<int32_t>(rc.x));
+      while (status == kDevOk && !(type == 0 && h == 0) && p0 >= T.r0 && j >= T.c0 + 1 &&
+             j < static_cast<int32_t>(n)) {
+        // the three reads are independent: issue them together
+        const int32_t hd_v = static_cast<int32_t>(T.m[(p0 - T.r0) * TC + (j - 1 - T.c0)]);
+        const uint32_t b_v = T.sq[j - T.c0];
+        const U32x4 rn = T.rec[max(p0 - T.rb, 0)];  // used only when p0 >= rb
+        // keeps the record's read next to the others (not sunk past the test)
+        __asm__ volatile("" ::"v"(rn.x), "v"(rn.y), "v"(rn.z));
+        const int32_t hd = U(hd_v);
+        const int32_t pr = Uniform(b_v) == Uniform(rc.y) ? prm.m : prm.mis;
+        if (h != hd + pr) break;
+        emit(U(static_cast<int32_t>(rc.z)), j);
+        i = p0;
+        --j;
+        h = hd;
+        rc = rn;
+        p0 = U(static_cast<int32_t>(rc.x));
+        moved = true;
+        if (i == 0 || i < T.rb) break;
+      }
+      if (moved) continue;
+    }
     const int32_t h = at(0, i, j);
     if (type == 0 && h == 0) break;
-    const uint32_t pb = gv.pred_off[i - 1], pe = gv.pred_off[i];
+    const uint32_t pb = pred_begin(i), pe = pred_end(i);
     const bool source = pb == pe;
     const uint32_t np_ = source ? 1u : pe - pb;
-    auto pred = [&](uint32_t k) -> int32_t { return source ? 0 : static_cast<int32_t>(gv.preds[pb + k]); };
+    auto pred = [&](uint32_t k) -> int32_t { return source ? 0 : pred_at(pb + k); };
     bool found = false, ext_left = false, ext_up = false;
     // diagonal (:1053-1063)
     const int32_t pr = prof(i, j);
@@ -281,7 +447,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
         found = true;
       }
     }
-    emit(i == prev_i ? -1 : gv.node_id[i - 1], j == prev_j ? -1 : j);
+    emit(i == prev_i ? -1 : node_of(i), j == prev_j ? -1 : j);
     i = prev_i;
     j = prev_j;
 
@@ -291,20 +457,22 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
         emit(-1, j);
         --j;
         if (j == -1) break;
+        ensure(i, j);
         if (at(2, i, j) + e != at(2, i, j + 1)) break;
         if (SUB == 2 && at(4, i, j) + c != at(4, i, j + 1)) break;
       }
     } else if (ext_up) {
       // :1510-1547 affine, :2040-2093 convex
       while (i != 0 && status == kDevOk) {
-        const uint32_t qb = gv.pred_off[i - 1], qe = gv.pred_off[i];
+        ensure(i, j);
+        const uint32_t qb = pred_begin(i), qe = pred_end(i);
         const int32_t fv = at(1, i, j);
         bool stop;
         prev_i = 0;
         if (SUB == 1) {
           stop = false;
           for (uint32_t k = qb; k < qe; ++k) {
-            const int32_t p = static_cast<int32_t>(gv.preds[k]);
+            const int32_t p = pred_at(k);
             if ((stop = (fv == at(0, p, j) + g)) || fv == at(1, p, j) + e) {
               prev_i = p;
               break;
@@ -314,7 +482,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
           const int32_t ov = at(3, i, j);
           stop = true;
           for (uint32_t k = qb; k < qe; ++k) {
-            const int32_t p = static_cast<int32_t>(gv.preds[k]);
+            const int32_t p = pred_at(k);
             if (fv == at(1, p, j) + e || ov == at(3, p, j) + c) {
               prev_i = p;
               stop = false;
@@ -323,7 +491,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
           }
           if (stop) {
             for (uint32_t k = qb; k < qe; ++k) {
-              const int32_t p = static_cast<int32_t>(gv.preds[k]);
+              const int32_t p = pred_at(k);
               if (fv == at(0, p, j) + g || ov == at(0, p, j) + q) {
                 prev_i = p;
                 break;
@@ -331,7 +499,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
             }
           }
         }
-        emit(gv.node_id[i - 1], -1);
+        emit(node_of(i), -1);
         i = prev_i;
         if (stop || i == 0) break;
       }
@@ -373,6 +541,12 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
       }
       i = next;
     }
+  }
+  // the last, partial batch of pairs
+  const uint32_t kept = min(np, cap);
+  if ((kept & 63u) != 0u) {
+    const uint32_t k = (kept & ~63u) + static_cast<uint32_t>(lane);
+    if (k < kept) *reinterpret_cast<int2*>(out + 2ull * k) = make_int2(buf_a, buf_b);
   }
   *n_out = np;
   *status_out = status;
@@ -1362,23 +1536,28 @@ PoaAlignKernel(const KArgs a) {
       __syncthreads();
     }
 
-    // ---- traceback in one lane
-    if (tid == 0) {
+    // ---- traceback by wave 0 (the query profile's LDS is free now)
+    if (wave == 0) {
       int32_t* res = a.results + 4 * jx;
       if (!found) {
-        res[0] = 0;
-        res[1] = 0;
-        res[2] = 0;
-        res[3] = kDevOk;
+        if (lane == 0) {
+          res[0] = 0;
+          res[1] = 0;
+          res[2] = 0;
+          res[3] = kDevOk;
+        }
       } else {
         uint32_t np = 0;
         int32_t status = kDevOk;
         Traceback<SUB, ST>(prm, gv, seq, n, mat, rstride, stride, max_i, max_j, a.pairs + 2ull * J.pair_off,
-                           J.cap, &np, &status);
-        res[0] = static_cast<int32_t>(np);
-        res[1] = max_score;
-        res[2] = 1;
-        res[3] = status;
+                           J.cap, &np, &status, (Lds<uint32_t>*)(s_prof),
+                           8u * K * static_cast<uint32_t>(nthreads), lane);
+        if (lane == 0) {
+          res[0] = static_cast<int32_t>(np);
+          res[1] = max_score;
+          res[2] = 1;
+          res[3] = status;
+        }
       }
     }
     __syncthreads();
