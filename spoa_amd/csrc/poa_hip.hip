@@ -310,7 +310,7 @@ struct spoa_hip_engine {
   } pending;
   PinBuf h_in, h_out;
   spoa_hip_stats stats{};
-  std::map<std::pair<KernelFn, int>, int> occupancy;
+  std::map<std::tuple<KernelFn, int, uint32_t>, int> occupancy;  // (kernel, threads, LDS) -> blocks a CU
   // overrides: (width, batch) -> max_width -> cfg
   std::map<std::pair<int, int>, std::map<uint32_t, Cfg>> overrides;
 };
@@ -559,18 +559,36 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     gp.threads = 64 * plans[x].cfg.waves;
     gp.lds = 8u * static_cast<uint32_t>(plans[x].cfg.k) * static_cast<uint32_t>(gp.threads);
     if (!gp.fn) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] unsupported launch configuration");
-    int per_cu = 0;
-    auto key = std::make_pair(gp.fn, gp.threads);
-    auto oc = E->occupancy.find(key);
-    if (oc == E->occupancy.end()) {
-      if (!HipOk(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gp.fn, gp.threads, gp.lds), "occupancy")) {
-        return SPOA_HIP_DEVICE_ERROR;
+    auto blocks_per_cu = [&](uint32_t lds, int* out) -> bool {
+      auto key = std::make_tuple(gp.fn, gp.threads, lds);
+      auto oc = E->occupancy.find(key);
+      if (oc != E->occupancy.end()) {
+        *out = oc->second;
+        return true;
       }
-      E->occupancy[key] = per_cu;
-    } else {
-      per_cu = oc->second;
-    }
+      if (!HipOk(hipOccupancyMaxActiveBlocksPerMultiprocessor(out, gp.fn, gp.threads, lds), "occupancy")) return false;
+      E->occupancy[key] = *out;
+      return true;
+    };
+    int per_cu = 0;
+    if (!blocks_per_cu(gp.lds, &per_cu)) return SPOA_HIP_DEVICE_ERROR;
     const uint32_t n = static_cast<uint32_t>(gp.b - gp.a);
+    // When the chip holds every job of the group at once, each workgroup
+    // may take more LDS (the traceback's tile grows with it) as long as
+    // they all still fit.
+    const uint32_t need = (n + static_cast<uint32_t>(E->num_cus) - 1) / static_cast<uint32_t>(E->num_cus);
+    if (need <= static_cast<uint32_t>(per_cu)) {
+      for (uint32_t cand : {65536u, 49152u, 40960u, 32768u, 24576u}) {
+        if (cand <= gp.lds) break;
+        int pc = 0;
+        if (!blocks_per_cu(cand, &pc)) return SPOA_HIP_DEVICE_ERROR;
+        if (static_cast<uint32_t>(pc) >= need) {
+          gp.lds = cand;
+          per_cu = pc;
+          break;
+        }
+      }
+    }
     gp.grid = std::max<uint32_t>(
         1u, std::min<uint32_t>(n, static_cast<uint32_t>(std::max(1, per_cu)) * static_cast<uint32_t>(E->num_cus)));
     gp.prm = MakeParams(P, plans[x].width, plans[x].cfg.k);
@@ -606,6 +624,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     args.seqs = din + off_seq;
     args.pairs = reinterpret_cast<int32_t*>(dout + off_pairs);
     args.results = reinterpret_cast<int32_t*>(dout + off_res) + 4 * gp.a;
+    args.lds_bytes = gp.lds;
     return args;
   };
   E->pass_width.clear();

@@ -75,26 +75,41 @@ struct DevJob {
 //   node_id[V], pred_off[V + 1] (relative to preds), fcH[V + 1],
 //   fcF[V + 1], fcO[V + 1], preds[...] (predecessor rows, rank + 1, in
 //   Node::inedges order).
+// The graph block is never written while a kernel runs, so it is read
+// through the constant address space: loads of one address for the whole
+// wave become scalar loads, which are counted apart from the vector stores
+// of the rows (waiting for a record or a predecessor index never waits for
+// those stores).
+template <typename T>
+using Cst = const __attribute__((address_space(4))) T;
+typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
+
 struct GraphView {
-  const uint4* rows;
-  const int32_t* node_id;
-  const uint32_t* pred_off;
-  const int32_t* fcH;
-  const int32_t* fcF;
-  const int32_t* fcO;
-  const uint32_t* preds;
+  Cst<U32x4>* rows;
+  Cst<int32_t>* node_id;
+  Cst<uint32_t>* pred_off;
+  Cst<int32_t>* fcH;
+  Cst<int32_t>* fcF;
+  Cst<int32_t>* fcO;
+  Cst<uint32_t>* preds;
 };
 
-__device__ __forceinline__ GraphView MakeView(const uint32_t* base, uint32_t V) {
+__device__ __forceinline__ GraphView MakeView(const uint32_t* base_g, uint32_t V) {
+  Cst<uint32_t>* base = (Cst<uint32_t>*)base_g;
   GraphView g;
-  g.rows = reinterpret_cast<const uint4*>(base);
-  g.node_id = reinterpret_cast<const int32_t*>(base + 4 * V);
+  g.rows = reinterpret_cast<Cst<U32x4>*>(base);
+  g.node_id = reinterpret_cast<Cst<int32_t>*>(base + 4 * V);
   g.pred_off = base + 5 * V;
-  g.fcH = reinterpret_cast<const int32_t*>(base + 6 * V + 1);
+  g.fcH = reinterpret_cast<Cst<int32_t>*>(base + 6 * V + 1);
   g.fcF = g.fcH + (V + 1);
   g.fcO = g.fcF + (V + 1);
   g.preds = base + 9 * V + 4;
   return g;
+}
+
+__device__ __forceinline__ uint4 LoadRow(const GraphView& g, uint32_t r) {
+  const U32x4 v = g.rows[r];
+  return make_uint4(v.x, v.y, v.z, v.w);
 }
 
 struct KArgs {
@@ -106,6 +121,7 @@ struct KArgs {
   const uint8_t* seqs;
   int32_t* pairs;            // (node id, column) pairs in traceback order
   int32_t* results;          // per job: n_pairs, score, has_score, status
+  uint32_t lds_bytes;        // dynamic LDS of the launch (at least the query profile's)
 };
 
 enum DevStatus : int32_t { kDevOk = 0, kDevCapacity = 6, kDevInconsistent = 7 };
@@ -197,8 +213,6 @@ __device__ __forceinline__ void LoadChars(const uint8_t* __restrict__ seq, uint3
 // stores them.
 template <typename T>
 using Lds = __attribute__((address_space(3))) T;
-typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
-
 // Device-memory reads of the walk outside its tile (long edges), kept out
 // of line so that the tile reads stay LDS reads.
 template <typename T>
@@ -208,12 +222,12 @@ __device__ __attribute__((noinline)) T TbLoad(const T* p) {
 
 template <typename ST>
 struct TbTile {
-  Lds<ST>* m;         // [NM][R][C]
+  Lds<ST>* m;         // [R][NM][C]
   Lds<uint32_t>* po;  // [R + 1] predecessor offsets of rows rb..r1 (and the end of r1)
   Lds<U32x4>* rec;    // [R] per row: first predecessor (0 for a source), base, node id
   Lds<uint32_t>* pr;  // [P] predecessors pb0 .. pb0 + pn
   Lds<uint8_t>* sq;   // [C] read bases of the tile columns
-  int32_t R = 0, C = 0, P = 0, margin = 0;
+  int32_t R = 0, C = 0, cshift = 0, P = 0, margin = 0;
   bool on = false;
   int32_t r0 = 1, r1 = 0, rb = 1, c0 = 0;
   uint32_t pb0 = 0, pn = 0;
@@ -226,26 +240,40 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
                           uint32_t* n_out, int32_t* status_out, Lds<uint32_t>* lds, uint32_t lds_bytes, int lane) {
   constexpr int NM = SUB == 0 ? 1 : (SUB == 1 ? 3 : 5);
   constexpr int S = sizeof(ST);
-  constexpr int VEC = 16 / S;          // columns a 16-byte load
-  constexpr int TC = S == 2 ? 32 : 24;  // tile columns
+  constexpr int VEC = 16 / S;  // columns a 16-byte load
   const int32_t g = prm.g, e = prm.e, q = prm.q, c = prm.c;
   const int type = prm.type;
   auto U = [](int32_t v) -> int32_t { return static_cast<int32_t>(Uniform(static_cast<uint32_t>(v))); };
 
-  // tile geometry from the LDS the kernel lends (its query profile, dead now)
+  // Tile geometry from the LDS the kernel lends (its query profile, dead
+  // now): a power-of-two column count C and as many rows as fit, chosen so
+  // that the walk -- about max_i / max_j rows a column -- runs longest
+  // before it leaves the tile.
   TbTile<ST> T;
   {
-    // matrices, offset, record, ~3 predecessors
-    const uint32_t per_row = NM * TC * S + 4 + 16 + 3 * 4;
-    // 64 bytes spare: the end offset of the last row and the tile's bases
-    const int32_t R = static_cast<int32_t>((lds_bytes > 64 ? lds_bytes - 64 : 0) / per_row);
-    if (R >= 8 && stride >= static_cast<uint32_t>(TC)) {
-      T.R = R;
-      T.C = TC;
+    const uint32_t avail = lds_bytes > 128 ? lds_bytes - 128 : 0;  // spare: end offset, bases
+    const int32_t ratio16 = min(64, max(16, static_cast<int32_t>((16ll * max(max_i, 1)) / (max_j + 1))));
+    int32_t best = -1;
+    for (int32_t cs = 4; cs <= 6; ++cs) {  // C = 16, 32, 64
+      const int32_t C = 1 << cs;
+      if (C < 2 * VEC || C > static_cast<int32_t>(stride)) continue;
+      // matrices, offset, record, ~3 predecessors a row
+      const int32_t R = static_cast<int32_t>(avail / static_cast<uint32_t>(NM * C * S + 4 + 16 + 3 * 4));
+      if (R < 8) continue;
+      const int32_t steps = min(C - VEC - 2, ((R - 4) * 16) / ratio16);
+      if (steps > best) {
+        best = steps;
+        T.R = R;
+        T.C = C;
+        T.cshift = cs;
+      }
+    }
+    if (best > 0) {
+      const int32_t R = T.R;
       T.P = 3 * R;
       T.margin = 4;
       T.m = reinterpret_cast<Lds<ST>*>(lds);
-      T.rec = reinterpret_cast<Lds<U32x4>*>(lds + (NM * R * TC * S) / 4);
+      T.rec = reinterpret_cast<Lds<U32x4>*>(lds + (NM * R * T.C * S) / 4);
       T.po = reinterpret_cast<Lds<uint32_t>*>(T.rec + R);
       T.pr = T.po + R + 1;
       T.sq = reinterpret_cast<Lds<uint8_t>*>(T.pr + T.P);
@@ -261,14 +289,23 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     if (c0 + T.C > static_cast<int32_t>(stride)) c0 = static_cast<int32_t>(stride) - T.C;
     T.c0 = c0;
     const int32_t nrow = T.r1 - T.r0 + 1;
-    constexpr int32_t kPer = TC / VEC;  // 16-byte pieces of one row of one matrix
-    const int32_t pieces = nrow * NM * kPer;
-    for (int32_t x = lane; x < pieces; x += 64) {
-      const int32_t k = x % kPer, rt = x / kPer, t = rt % NM, r = rt / NM;
-      const U32x4 v = *reinterpret_cast<const U32x4*>(mat + static_cast<uint64_t>(T.r0 + r) * rstride +
-                                                      static_cast<uint64_t>(t) * stride +
-                                                      static_cast<uint32_t>(c0 + k * VEC));
-      *reinterpret_cast<Lds<U32x4>*>(T.m + (t * T.R + r) * TC + k * VEC) = v;
+    // 16-byte pieces of one row of one matrix: C / VEC = 1 << kps
+    const int32_t kps = T.cshift - (S == 2 ? 3 : 2);
+    const int32_t pieces = (nrow * NM) << kps;
+    // global -> LDS loads: lane l of a round writes 16 bytes at the round's
+    // LDS base + 16 l, so the tile is laid out in piece order: [row][matrix][column].
+    // Nothing is pending when the rounds start (a wait the compiler sees),
+    // so they issue back to back.
+    __builtin_amdgcn_s_waitcnt(0);
+    for (int32_t x0 = 0; x0 < pieces; x0 += 64) {
+      const int32_t x = x0 + lane;
+      if (x < pieces) {
+        const int32_t k = x & ((1 << kps) - 1), rt = x >> kps, t = rt % NM, r = rt / NM;
+        const ST* src = mat + static_cast<uint64_t>(T.r0 + r) * rstride + static_cast<uint64_t>(t) * stride +
+                        static_cast<uint32_t>(c0 + k * VEC);
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(src),
+                                         reinterpret_cast<Lds<void>*>(T.m + x0 * VEC), 16, 0, 0);
+      }
     }
     // rows rb..r1 carry graph data (row 0 has none)
     T.rb = max(T.r0, 1);
@@ -277,7 +314,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
       T.po[x] = gv.pred_off[T.rb - 1 + x];
       if (x < nm) {
         // the row record of the view holds the base and the first predecessor
-        const uint4 rw = gv.rows[T.rb - 1 + x];
+        const uint4 rw = LoadRow(gv, T.rb - 1 + x);
         T.rec[x] = U32x4{rw.y, rw.x & 0xffu, static_cast<uint32_t>(gv.node_id[T.rb - 1 + x]), 0u};
       }
     }
@@ -285,9 +322,9 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     const uint32_t tot = nm > 0 ? Uniform(gv.pred_off[T.r1]) - T.pb0 : 0u;
     T.pn = min(tot, static_cast<uint32_t>(T.P));
     for (uint32_t x = static_cast<uint32_t>(lane); x < T.pn; x += 64) T.pr[x] = gv.preds[T.pb0 + x];
-    if (lane < TC) T.sq[lane] = c0 + lane < static_cast<int32_t>(n) ? seq[c0 + lane] : 0;
+    if (lane < T.C) T.sq[lane] = c0 + lane < static_cast<int32_t>(n) ? seq[c0 + lane] : 0;
     // the wave's LDS writes land before any lane reads them
-    __asm__ volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
   };
   auto ensure = [&](int32_t i, int32_t j) {
@@ -302,7 +339,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
   // matrix t of row r at column col (t: 0 H, 1 F, 2 E, 3 O, 4 Q)
   auto at = [&](int t, int32_t r, int32_t col) -> int32_t {
     if (r >= T.r0 && r <= T.r1 && col >= T.c0 && col < T.c0 + T.C) {
-      return U(static_cast<int32_t>(T.m[(t * T.R + (r - T.r0)) * TC + (col - T.c0)]));
+      return U(static_cast<int32_t>(T.m[(((r - T.r0) * NM + t) << T.cshift) + (col - T.c0)]));
     }
     return U(static_cast<int32_t>(TbLoad(mat + static_cast<uint64_t>(r) * rstride + static_cast<uint64_t>(t) * stride +
                                          static_cast<uint32_t>(col))));
@@ -314,14 +351,23 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
   auto pred_end = [&](int32_t r) -> uint32_t {
     return Uniform(r >= T.rb && r <= T.r1 ? T.po[r - T.rb + 1] : TbLoad(gv.pred_off + r));
   };
-  auto pred_at = [&](uint32_t k) -> int32_t {
-    return U(static_cast<int32_t>(k - T.pb0 < T.pn ? T.pr[k - T.pb0] : TbLoad(gv.preds + k)));
+
+  // the same reads for a value that differs by lane
+  auto at_v = [&](int t, int32_t r, int32_t col) -> int32_t {
+    if (r >= T.r0 && r <= T.r1 && col >= T.c0 && col < T.c0 + T.C) {
+      return static_cast<int32_t>(T.m[(((r - T.r0) * NM + t) << T.cshift) + (col - T.c0)]);
+    }
+    return static_cast<int32_t>(
+        mat[static_cast<uint64_t>(r) * rstride + static_cast<uint64_t>(t) * stride + static_cast<uint32_t>(col)]);
+  };
+  auto pred_v = [&](uint32_t k) -> int32_t {
+    return static_cast<int32_t>(k - T.pb0 < T.pn ? T.pr[k - T.pb0] : gv.preds[k]);
   };
   auto node_of = [&](int32_t r) -> int32_t {
     return U(r >= T.rb && r <= T.r1 ? static_cast<int32_t>(T.rec[r - T.rb].z) : TbLoad(gv.node_id + (r - 1)));
   };
   auto base_of = [&](int32_t r) -> uint32_t {
-    return Uniform(r >= T.rb && r <= T.r1 ? T.rec[r - T.rb].y : (TbLoad(&gv.rows[r - 1].x) & 0xffu));
+    return Uniform(r >= T.rb && r <= T.r1 ? T.rec[r - T.rb].y : (LoadRow(gv, r - 1).x & 0xffu));
   };
   auto prof = [&](int32_t r, int32_t col) -> int32_t {
     if (col >= static_cast<int32_t>(n)) return prm.pad;
@@ -366,7 +412,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
       while (status == kDevOk && !(type == 0 && h == 0) && p0 >= T.r0 && j >= T.c0 + 1 &&
              j < static_cast<int32_t>(n)) {
         // the three reads are independent: issue them together
-        const int32_t hd_v = static_cast<int32_t>(T.m[(p0 - T.r0) * TC + (j - 1 - T.c0)]);
+        const int32_t hd_v = static_cast<int32_t>(T.m[(((p0 - T.r0) * NM) << T.cshift) + (j - 1 - T.c0)]);
         const uint32_t b_v = T.sq[j - T.c0];
         const U32x4 rn = T.rec[max(p0 - T.rb, 0)];  // used only when p0 >= rb
         // keeps the record's read next to the others (not sunk past the test)
@@ -385,45 +431,58 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
       }
       if (moved) continue;
     }
+    // The general step: one lane a predecessor, 64 at a time; the first
+    // hit in predecessor order wins, as in the serial loops.
     const int32_t h = at(0, i, j);
     if (type == 0 && h == 0) break;
     const uint32_t pb = pred_begin(i), pe = pred_end(i);
     const bool source = pb == pe;
-    const uint32_t np_ = source ? 1u : pe - pb;
-    auto pred = [&](uint32_t k) -> int32_t { return source ? 0 : pred_at(pb + k); };
+    const uint32_t npr = source ? 1u : pe - pb;
     bool found = false, ext_left = false, ext_up = false;
     // diagonal (:1053-1063)
     const int32_t pr = prof(i, j);
-    for (uint32_t k = 0; k < np_; ++k) {
-      const int32_t p = pred(k);
-      const int32_t hd = j == 0 ? gv.fcH[p] : at(0, p, j - 1);
-      if (h == hd + pr) {
-        prev_i = p;
+    for (uint32_t k0 = 0; k0 < npr && !found; k0 += 64) {
+      const uint32_t k = k0 + static_cast<uint32_t>(lane);
+      const bool act = k < npr;
+      const int32_t p = act && !source ? pred_v(pb + k) : 0;
+      bool hit = false;
+      if (act) hit = h == (j == 0 ? gv.fcH[p] : at_v(0, p, j - 1)) + pr;
+      const uint64_t hm = __ballot(hit);
+      if (hm != 0) {
+        prev_i = __builtin_amdgcn_readlane(p, static_cast<int>(__builtin_ctzll(hm)));
         prev_j = j - 1;
         found = true;
-        break;
       }
     }
-    // up (:1065-1074, :1447-1457, :1963-1975)
-    if (!found) {
-      for (uint32_t k = 0; k < np_; ++k) {
-        const int32_t p = pred(k);
-        const int32_t hu = at(0, p, j);
-        bool hit;
+    // up (:1065-1074, :1447-1457, :1963-1975); the gap extends where the
+    // winning test was an extension
+    for (uint32_t k0 = 0; k0 < npr && !found; k0 += 64) {
+      const uint32_t k = k0 + static_cast<uint32_t>(lane);
+      const bool act = k < npr;
+      const int32_t p = act && !source ? pred_v(pb + k) : 0;
+      bool hit = false, ext = false;
+      if (act) {
+        const int32_t hu = at_v(0, p, j);
         if (SUB == 0) {
           hit = h == hu + g;
         } else if (SUB == 1) {
-          hit = (ext_up = (h == at(1, p, j) + e)) || h == hu + g;
+          const bool fh = h == at_v(1, p, j) + e;
+          hit = fh || h == hu + g;
+          ext = fh;
         } else {
-          hit = (ext_up = (h == at(1, p, j) + e)) || h == hu + g || (ext_up = (h == at(3, p, j) + c)) ||
-                h == hu + q;
+          const bool fh = h == at_v(1, p, j) + e, gh = h == hu + g, oh = h == at_v(3, p, j) + c,
+                     qh = h == hu + q;
+          hit = fh || gh || oh || qh;
+          ext = fh || (!gh && oh);
         }
-        if (hit) {
-          prev_i = p;
-          prev_j = j;
-          found = true;
-          break;
-        }
+      }
+      const uint64_t hm = __ballot(hit);
+      if (hm != 0) {
+        const int w = static_cast<int>(__builtin_ctzll(hm));
+        prev_i = __builtin_amdgcn_readlane(p, w);
+        ext_up = __builtin_amdgcn_readlane(ext ? 1 : 0, w) != 0;
+        prev_j = j;
+        found = true;
       }
     }
     // left (:1076-1083, :1459-1468, :1977-1990); column 0 reads the first column
@@ -452,48 +511,82 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     j = prev_j;
 
     if (ext_left) {
-      // :1484-1509, :2006-2039 -- convex stops when either chain breaks
-      while (status == kDevOk) {
-        emit(-1, j);
-        --j;
-        if (j == -1) break;
-        ensure(i, j);
-        if (at(2, i, j) + e != at(2, i, j + 1)) break;
-        if (SUB == 2 && at(4, i, j) + c != at(4, i, j + 1)) break;
+      // :1484-1509, :2006-2039 -- convex stops when either chain breaks.
+      // The walk emits columns j, j - 1, ... and stops at the first column
+      // jj < j (going left) that is -1 or where the chain breaks; the
+      // chain is tested 64 columns at a time.
+      int32_t jj = j - 1;
+      while (true) {
+        const int32_t col = jj - lane;
+        bool brk = col < 0;
+        if (!brk) {
+          brk = at_v(2, i, col) + e != at_v(2, i, col + 1) ||
+                (SUB == 2 && at_v(4, i, col) + c != at_v(4, i, col + 1));
+        }
+        const uint64_t bm = __ballot(brk);
+        if (bm != 0) {
+          jj -= static_cast<int32_t>(__builtin_ctzll(bm));
+          break;
+        }
+        jj -= 64;
       }
+      for (int32_t col = j; col > jj; --col) {
+        emit(-1, col);
+        if (status != kDevOk) {
+          jj = col - 1;
+          break;
+        }
+      }
+      j = jj;
     } else if (ext_up) {
       // :1510-1547 affine, :2040-2093 convex
       while (i != 0 && status == kDevOk) {
         ensure(i, j);
         const uint32_t qb = pred_begin(i), qe = pred_end(i);
+        const uint32_t nq = qe - qb;
         const int32_t fv = at(1, i, j);
         bool stop;
         prev_i = 0;
         if (SUB == 1) {
           stop = false;
-          for (uint32_t k = qb; k < qe; ++k) {
-            const int32_t p = pred_at(k);
-            if ((stop = (fv == at(0, p, j) + g)) || fv == at(1, p, j) + e) {
-              prev_i = p;
+          for (uint32_t k0 = 0; k0 < nq; k0 += 64) {
+            const uint32_t k = k0 + static_cast<uint32_t>(lane);
+            const int32_t p = k < nq ? pred_v(qb + k) : 0;
+            const bool gh = k < nq && fv == at_v(0, p, j) + g;
+            const bool fh = k < nq && fv == at_v(1, p, j) + e;
+            const uint64_t hm = __ballot(gh || fh);
+            if (hm != 0) {
+              const int w = static_cast<int>(__builtin_ctzll(hm));
+              prev_i = __builtin_amdgcn_readlane(p, w);
+              stop = __builtin_amdgcn_readlane(gh ? 1 : 0, w) != 0;
               break;
             }
           }
         } else {
           const int32_t ov = at(3, i, j);
           stop = true;
-          for (uint32_t k = qb; k < qe; ++k) {
-            const int32_t p = pred_at(k);
-            if (fv == at(1, p, j) + e || ov == at(3, p, j) + c) {
-              prev_i = p;
+          for (uint32_t k0 = 0; k0 < nq && stop; k0 += 64) {
+            const uint32_t k = k0 + static_cast<uint32_t>(lane);
+            const int32_t p = k < nq ? pred_v(qb + k) : 0;
+            const bool hit = k < nq && (fv == at_v(1, p, j) + e || ov == at_v(3, p, j) + c);
+            const uint64_t hm = __ballot(hit);
+            if (hm != 0) {
+              prev_i = __builtin_amdgcn_readlane(p, static_cast<int>(__builtin_ctzll(hm)));
               stop = false;
-              break;
             }
           }
           if (stop) {
-            for (uint32_t k = qb; k < qe; ++k) {
-              const int32_t p = pred_at(k);
-              if (fv == at(0, p, j) + g || ov == at(0, p, j) + q) {
-                prev_i = p;
+            for (uint32_t k0 = 0; k0 < nq; k0 += 64) {
+              const uint32_t k = k0 + static_cast<uint32_t>(lane);
+              const int32_t p = k < nq ? pred_v(qb + k) : 0;
+              bool hit = false;
+              if (k < nq) {
+                const int32_t hv = at_v(0, p, j);
+                hit = fv == hv + g || ov == hv + q;
+              }
+              const uint64_t hm = __ballot(hit);
+              if (hm != 0) {
+                prev_i = __builtin_amdgcn_readlane(p, static_cast<int>(__builtin_ctzll(hm)));
                 break;
               }
             }
@@ -871,8 +964,8 @@ PoaAlignKernel(const KArgs a) {
     // row records two ahead, and the first predecessor row of the next row
     // when it is not the current one: loaded before the current row's
     // stores, so waiting for it never waits for those stores
-    uint4 rec = gv.rows[0];
-    uint4 rec2 = V > 1 ? gv.rows[1] : make_uint4(0u, 0u, 0u, 0u);
+    uint4 rec = LoadRow(gv, 0);
+    uint4 rec2 = V > 1 ? LoadRow(gv, 1) : make_uint4(0u, 0u, 0u, 0u);
     uint32_t pfH[KW], pfF[KW], pfO[KW];
     bool pf_valid = false;
     uint32_t pf_row = 0;
@@ -890,7 +983,7 @@ PoaAlignKernel(const KArgs a) {
       const uint4 cur = rec;
       const uint4 nxt = rec2;  // row r + 1 (when r < V)
       rec = rec2;
-      if (r + 1 < V) rec2 = gv.rows[r + 1];
+      if (r + 1 < V) rec2 = LoadRow(gv, r + 1);
       const uint32_t rch = cur.x & 0xffu;
       const bool sink = ((cur.x >> 8) & 1u) != 0;
       const uint32_t slot = (cur.x >> 9) & 7u;
@@ -1551,7 +1644,7 @@ PoaAlignKernel(const KArgs a) {
         int32_t status = kDevOk;
         Traceback<SUB, ST>(prm, gv, seq, n, mat, rstride, stride, max_i, max_j, a.pairs + 2ull * J.pair_off,
                            J.cap, &np, &status, (Lds<uint32_t>*)(s_prof),
-                           8u * K * static_cast<uint32_t>(nthreads), lane);
+                           a.lds_bytes, lane);
         if (lane == 0) {
           res[0] = static_cast<int32_t>(np);
           res[1] = max_score;
