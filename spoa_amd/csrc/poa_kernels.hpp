@@ -174,6 +174,13 @@ __device__ __forceinline__ int32_t Dpp(int32_t old, int32_t v) {
   return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
 }
 
+// Waits for every vector memory operation of the wave (vmcnt 0), through
+// the builtin so that the compiler knows the loads before it have landed.
+// Vector loads and stores share one in-order counter: a load issued before
+// a row's stores and waited for after them would wait for the stores too,
+// so every load is waited for before the stores that follow it.
+__device__ __forceinline__ void WaitVmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Barrier for LDS traffic only: waits for this wave's LDS operations, not
 // for its global loads and stores (which __syncthreads would also drain).
 __device__ __forceinline__ void LdsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -1025,15 +1032,32 @@ PoaAlignKernel(const KArgs a) {
           const bool cached = single && p + 1 == r;
           const bool fetched = !cached && pi == 0 && pf_valid && pf_row == p;
           const __amdgpu_buffer_rsrc_t rp = RowRsrc(row_ptr(p), rbytes);
-          uint32_t hv[KW];
+          // the predecessor's H (and F, O): kept in registers, prefetched,
+          // or read now -- all reads of a row together, then waited for
+          uint32_t hv[KW], fp[KW], op[KW];
+          int32_t hb = 0;  // lane 0: H_p left of the segment, read from bnd
           if (cached) {
 #pragma unroll
-            for (int i = 0; i < KW; ++i) hv[i] = cH[i];
+            for (int i = 0; i < KW; ++i) {
+              hv[i] = cH[i];
+              if (SUB >= 1) fp[i] = cF[i];
+              if (SUB == 2) op[i] = cO[i];
+            }
           } else if (fetched) {
 #pragma unroll
-            for (int i = 0; i < KW; ++i) hv[i] = pfH[i];
+            for (int i = 0; i < KW; ++i) {
+              hv[i] = pfH[i];
+              if (SUB >= 1) fp[i] = pfF[i];
+              if (SUB == 2) op[i] = pfO[i];
+            }
           } else {
             BufLoad<KW>(rp, active ? off : rbytes, hv);
+            if (SUB >= 1) BufLoad<KW>(rp, active ? off + mbytes : rbytes, fp);
+            if (SUB == 2) BufLoad<KW>(rp, active ? off + 3 * mbytes : rbytes, op);
+            if (lane == 0 && c0 != 0 && p != 0) {
+              hb = bnd[static_cast<uint64_t>(t * static_cast<uint32_t>(nwaves) + wave) * (V + 1) + p];
+            }
+            WaitVmem();
           }
           // H_p[c0 - 1]: neighbouring lane, or the first column / memory
           // across waves and tiles
@@ -1048,7 +1072,7 @@ PoaAlignKernel(const KArgs a) {
             } else if (fetched) {
               hm1 = pf_hm1;
             } else {
-              hm1 = bnd[static_cast<uint64_t>(t * static_cast<uint32_t>(nwaves) + wave) * (V + 1) + p];
+              hm1 = hb;
             }
           }
           // diagonal (:838-850): H_p[c - 1] + profile
@@ -1071,29 +1095,9 @@ PoaAlignKernel(const KArgs a) {
 #pragma unroll
             for (int i = 0; i < KW; ++i) vf[i] = WAdd<ST>(hv[i], G2);
           } else {
-            uint32_t fp[KW];
-            if (cached) {
-#pragma unroll
-              for (int i = 0; i < KW; ++i) fp[i] = cF[i];
-            } else if (fetched) {
-#pragma unroll
-              for (int i = 0; i < KW; ++i) fp[i] = pfF[i];
-            } else {
-              BufLoad<KW>(rp, active ? off + mbytes : rbytes, fp);
-            }
 #pragma unroll
             for (int i = 0; i < KW; ++i) vf[i] = WMax<ST>(WAdd<ST>(hv[i], G2), WAdd<ST>(fp[i], E2));
             if (SUB == 2) {
-              uint32_t op[KW];
-              if (cached) {
-#pragma unroll
-                for (int i = 0; i < KW; ++i) op[i] = cO[i];
-              } else if (fetched) {
-#pragma unroll
-                for (int i = 0; i < KW; ++i) op[i] = pfO[i];
-              } else {
-                BufLoad<KW>(rp, active ? off + 3 * mbytes : rbytes, op);
-              }
 #pragma unroll
               for (int i = 0; i < KW; ++i) vo[i] = WMax<ST>(WAdd<ST>(hv[i], Q2), WAdd<ST>(op[i], C2));
             }
@@ -1151,10 +1155,6 @@ PoaAlignKernel(const KArgs a) {
             pf_valid = true;
             pf_row = q;
           }
-        }
-        if (SUB >= 1 && active) {
-          BufStore<KW>(rrow, off + mbytes, fv);
-          if (SUB == 2) BufStore<KW>(rrow, off + 3 * mbytes, ov);
         }
         if (single) {
 #pragma unroll
@@ -1374,7 +1374,15 @@ PoaAlignKernel(const KArgs a) {
 
         // ---- 3. fix-up: every column's value is the better of its local
         // value and the carry moved to it; then H.  E / Q leave in 16-byte
-        // chunks as soon as a chunk is complete, H after the row.
+        // chunks as soon as a chunk is complete, H after the row.  The
+        // prefetch has had the row's work to land; it is waited for here,
+        // before the row's first store (F, O), so that the next row never
+        // waits for this row's stores.
+        WaitVmem();
+        if (SUB >= 1 && active) {
+          BufStore<KW>(rrow, off + mbytes, fv);
+          if (SUB == 2) BufStore<KW>(rrow, off + 3 * mbytes, ov);
+        }
         uint32_t hw[KW];
         if constexpr (CPW == 2) {
           if constexpr (SUB == 0) {
