@@ -279,6 +279,11 @@ struct spoa_hip_engine {
   // that their launches run one after another and never share the device
   hipStream_t compute = nullptr;
   hipStream_t compute_own = nullptr;  // created when other engines share this one's kernels
+  // launch groups after a pass's first run on this stream, beside the first
+  // (for example the few int32 jobs of a pass beside its int16 jobs)
+  hipStream_t side = nullptr;
+  hipStream_t side_own = nullptr;
+  hipEvent_t ev_reset = nullptr, ev_side = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;  // copy timing
   hipEvent_t ev_in = nullptr, ev_kdone = nullptr;  // copy -> kernels -> copy ordering
   std::vector<hipEvent_t> ev_ks, ev_ke;            // start / end of every launch of a pass
@@ -675,14 +680,26 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     }
     spoa_hip_dev::ResetCounters<<<1, 256, 0, cs>>>(counters, static_cast<uint32_t>(groups.size()));
     if (!HipOk(hipGetLastError(), "counter reset")) return SPOA_HIP_DEVICE_ERROR;
-    // the groups run one after another on the compute stream, each timed
+    // The first group runs on the compute stream, the others beside it on
+    // the side stream (one after another there), each launch timed by its
+    // own events; the copy out waits for both streams.
+    const bool beside = groups.size() > 1 && E->side != cs;
+    if (beside && (!HipOk(hipEventRecord(E->ev_reset, cs), "event") ||
+                   !HipOk(hipStreamWaitEvent(E->side, E->ev_reset, 0), "stream wait"))) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
     for (size_t gi = 0; gi < groups.size(); ++gi) {
       const Group& gp = groups[gi];
-      if (!HipOk(hipEventRecord(E->ev_ks[gi], cs), "event")) return SPOA_HIP_DEVICE_ERROR;
-      gp.fn<<<gp.grid, gp.threads, gp.lds, cs>>>(args_of(gi));
-      if (!HipOk(hipGetLastError(), "kernel launch") || !HipOk(hipEventRecord(E->ev_ke[gi], cs), "event")) {
+      hipStream_t ks = gi > 0 && beside ? E->side : cs;
+      if (!HipOk(hipEventRecord(E->ev_ks[gi], ks), "event")) return SPOA_HIP_DEVICE_ERROR;
+      gp.fn<<<gp.grid, gp.threads, gp.lds, ks>>>(args_of(gi));
+      if (!HipOk(hipGetLastError(), "kernel launch") || !HipOk(hipEventRecord(E->ev_ke[gi], ks), "event")) {
         return SPOA_HIP_DEVICE_ERROR;
       }
+    }
+    if (beside && (!HipOk(hipEventRecord(E->ev_side, E->side), "event") ||
+                   !HipOk(hipStreamWaitEvent(cs, E->ev_side, 0), "stream wait"))) {
+      return SPOA_HIP_DEVICE_ERROR;
     }
     E->pass_launches = groups.size();
     if (cs != st && (!HipOk(hipEventRecord(E->ev_kdone, cs), "event") ||
@@ -934,12 +951,16 @@ int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_e
       !HipOk(hipEventCreate(&E->ev0), "hipEventCreate") || !HipOk(hipEventCreate(&E->ev1), "hipEventCreate") ||
       !HipOk(hipEventCreate(&E->ev2), "hipEventCreate") || !HipOk(hipEventCreate(&E->ev3), "hipEventCreate") ||
       !HipOk(hipEventCreateWithFlags(&E->ev_in, hipEventDisableTiming), "hipEventCreate") ||
-      !HipOk(hipEventCreateWithFlags(&E->ev_kdone, hipEventDisableTiming), "hipEventCreate")) {
+      !HipOk(hipEventCreateWithFlags(&E->ev_kdone, hipEventDisableTiming), "hipEventCreate") ||
+      !HipOk(hipEventCreateWithFlags(&E->ev_reset, hipEventDisableTiming), "hipEventCreate") ||
+      !HipOk(hipEventCreateWithFlags(&E->ev_side, hipEventDisableTiming), "hipEventCreate") ||
+      !HipOk(hipStreamCreateWithFlags(&E->side_own, hipStreamNonBlocking), "hipStreamCreate")) {
     spoa_hip_engine_destroy(E);
     return SPOA_HIP_DEVICE_ERROR;
   }
   E->compute = E->stream;
   E->copy = E->stream;
+  E->side = E->side_own;
   *out = E;
   return SPOA_HIP_OK;
 }
@@ -964,6 +985,12 @@ int spoa_hip_engine_destroy(spoa_hip_engine* E) {
   if (E->graph) (void)hipGraphDestroy(E->graph);
   if (E->ev_in) (void)hipEventDestroy(E->ev_in);
   if (E->ev_kdone) (void)hipEventDestroy(E->ev_kdone);
+  if (E->ev_reset) (void)hipEventDestroy(E->ev_reset);
+  if (E->ev_side) (void)hipEventDestroy(E->ev_side);
+  if (E->side_own) {
+    (void)hipStreamSynchronize(E->side_own);
+    (void)hipStreamDestroy(E->side_own);
+  }
   for (auto ev : E->ev_ks) (void)hipEventDestroy(ev);
   for (auto ev : E->ev_ke) (void)hipEventDestroy(ev);
   if (E->compute_own) {
@@ -1111,6 +1138,7 @@ int spoa_hip_share_compute(spoa_hip_engine* E, spoa_hip_engine* owner) {
     owner->compute = owner->compute_own;
   }
   E->compute = owner ? owner->compute : E->stream;
+  E->side = owner ? owner->side : E->side_own;
   return SPOA_HIP_OK;
 }
 
@@ -1121,6 +1149,7 @@ int spoa_hip_share_stream(spoa_hip_engine* E, spoa_hip_engine* owner) {
   }
   if (E->pending.active) (void)RunPassFinish(E);
   E->compute = E->copy = owner ? owner->stream : E->stream;
+  E->side = owner ? owner->stream : E->side_own;
   return SPOA_HIP_OK;
 }
 
