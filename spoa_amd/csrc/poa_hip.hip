@@ -281,8 +281,12 @@ struct spoa_hip_engine {
   hipStream_t compute_own = nullptr;  // created when other engines share this one's kernels
   // launch groups after a pass's first run on this stream, beside the first
   // (for example the few int32 jobs of a pass beside its int16 jobs)
+  // (made on first use: streams map onto the device's few hardware queues
+  // in creation order, and engines that never need one must not shift
+  // that mapping for the streams made after them)
   hipStream_t side = nullptr;
   hipStream_t side_own = nullptr;
+  spoa_hip_engine* side_owner = nullptr;  // whose side stream this engine shares (null: its own)
   hipEvent_t ev_reset = nullptr, ev_side = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;  // copy timing
   hipEvent_t ev_in = nullptr, ev_kdone = nullptr;  // copy -> kernels -> copy ordering
@@ -683,6 +687,13 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     // The first group runs on the compute stream, the others beside it on
     // the side stream (one after another there), each launch timed by its
     // own events; the copy out waits for both streams.
+    if (groups.size() > 1 && !E->side) {
+      spoa_hip_engine* so = E->side_owner ? E->side_owner : E;
+      if (!so->side_own && !HipOk(hipStreamCreateWithFlags(&so->side_own, hipStreamNonBlocking), "hipStreamCreate")) {
+        return SPOA_HIP_DEVICE_ERROR;
+      }
+      E->side = so->side_own;
+    }
     const bool beside = groups.size() > 1 && E->side != cs;
     if (beside && (!HipOk(hipEventRecord(E->ev_reset, cs), "event") ||
                    !HipOk(hipStreamWaitEvent(E->side, E->ev_reset, 0), "stream wait"))) {
@@ -953,14 +964,12 @@ int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_e
       !HipOk(hipEventCreateWithFlags(&E->ev_in, hipEventDisableTiming), "hipEventCreate") ||
       !HipOk(hipEventCreateWithFlags(&E->ev_kdone, hipEventDisableTiming), "hipEventCreate") ||
       !HipOk(hipEventCreateWithFlags(&E->ev_reset, hipEventDisableTiming), "hipEventCreate") ||
-      !HipOk(hipEventCreateWithFlags(&E->ev_side, hipEventDisableTiming), "hipEventCreate") ||
-      !HipOk(hipStreamCreateWithFlags(&E->side_own, hipStreamNonBlocking), "hipStreamCreate")) {
+      !HipOk(hipEventCreateWithFlags(&E->ev_side, hipEventDisableTiming), "hipEventCreate")) {
     spoa_hip_engine_destroy(E);
     return SPOA_HIP_DEVICE_ERROR;
   }
   E->compute = E->stream;
   E->copy = E->stream;
-  E->side = E->side_own;
   *out = E;
   return SPOA_HIP_OK;
 }
@@ -1138,7 +1147,8 @@ int spoa_hip_share_compute(spoa_hip_engine* E, spoa_hip_engine* owner) {
     owner->compute = owner->compute_own;
   }
   E->compute = owner ? owner->compute : E->stream;
-  E->side = owner ? owner->side : E->side_own;
+  E->side_owner = owner;
+  E->side = nullptr;
   return SPOA_HIP_OK;
 }
 
@@ -1149,7 +1159,9 @@ int spoa_hip_share_stream(spoa_hip_engine* E, spoa_hip_engine* owner) {
   }
   if (E->pending.active) (void)RunPassFinish(E);
   E->compute = E->copy = owner ? owner->stream : E->stream;
-  E->side = owner ? owner->stream : E->side_own;
+  // everything of this engine on the owner's one stream
+  E->side_owner = nullptr;
+  E->side = owner ? owner->stream : nullptr;
   return SPOA_HIP_OK;
 }
 
