@@ -85,6 +85,8 @@ typedef struct spoa_batch_stats {
   uint64_t cells, dp_bytes, align_calls;
   uint32_t launches, rounds;
   double kernel_ms16, kernel_ms32; /* kernel_ms of the int16 / int32 launches */
+  uint64_t dp_bytes16, dp_bytes32; /* dp_bytes of the int16 / int32 jobs */
+  uint32_t launches16, launches32; /* launches of int16 / int32 jobs */
 } spoa_batch_stats;
 
 /* Runs every window (reads [win_read_off[w], win_read_off[w+1]); read k is

@@ -137,6 +137,8 @@ typedef struct spoa_hip_stats {
   uint64_t dp_bytes;
   double kernel_ms16;  /* kernel_ms of the int16-score launches */
   double kernel_ms32;  /* kernel_ms of the int32-score launches */
+  uint64_t dp_bytes16, dp_bytes32;  /* dp_bytes of the int16 / int32 jobs */
+  uint32_t launches16, launches32;  /* launches of int16 / int32 jobs */
 } spoa_hip_stats;
 int spoa_hip_last_stats(spoa_hip_engine* engine, spoa_hip_stats* out);
 

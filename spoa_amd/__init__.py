@@ -44,7 +44,9 @@ class HipStats(ctypes.Structure):
                 ("unpack_ms", ctypes.c_double),
                 ("launches", ctypes.c_uint32), ("jobs", ctypes.c_uint32),
                 ("cells", ctypes.c_uint64), ("dp_bytes", ctypes.c_uint64),
-                ("kernel_ms16", ctypes.c_double), ("kernel_ms32", ctypes.c_double)]
+                ("kernel_ms16", ctypes.c_double), ("kernel_ms32", ctypes.c_double),
+                ("dp_bytes16", ctypes.c_uint64), ("dp_bytes32", ctypes.c_uint64),
+                ("launches16", ctypes.c_uint32), ("launches32", ctypes.c_uint32)]
 
 
 class BatchOpts(ctypes.Structure):
@@ -62,7 +64,9 @@ class BatchStats(ctypes.Structure):
                 ("unpack_ms", ctypes.c_double), ("cells", ctypes.c_uint64),
                 ("dp_bytes", ctypes.c_uint64), ("align_calls", ctypes.c_uint64),
                 ("launches", ctypes.c_uint32), ("rounds", ctypes.c_uint32),
-                ("kernel_ms16", ctypes.c_double), ("kernel_ms32", ctypes.c_double)]
+                ("kernel_ms16", ctypes.c_double), ("kernel_ms32", ctypes.c_double),
+                ("dp_bytes16", ctypes.c_uint64), ("dp_bytes32", ctypes.c_uint64),
+                ("launches16", ctypes.c_uint32), ("launches32", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -346,6 +350,10 @@ class Windows:
     @property
     def n_windows(self):
         return len(self.win_read_off) - 1
+
+    def max_read_len(self):
+        """Length of the longest read of every window."""
+        return int(np.diff(self.read_off).max()) if len(self.read_off) > 1 else 0
 
     def reads(self, w):
         lo, hi = int(self.win_read_off[w]), int(self.win_read_off[w + 1])

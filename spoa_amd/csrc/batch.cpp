@@ -218,6 +218,10 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
     st.kernel_ms += hs.kernel_ms;
     st.kernel_ms16 += hs.kernel_ms16;
     st.kernel_ms32 += hs.kernel_ms32;
+    st.dp_bytes16 += hs.dp_bytes16;
+    st.dp_bytes32 += hs.dp_bytes32;
+    st.launches16 += hs.launches16;
+    st.launches32 += hs.launches32;
     st.h2d_ms += hs.h2d_ms;
     st.d2h_ms += hs.d2h_ms;
     st.pack_ms += hs.host_ms;

@@ -350,6 +350,10 @@ int spoa_batch_get_stats(void* h, spoa_batch_stats* out) {
   out->rounds = s.rounds;
   out->kernel_ms16 = s.kernel_ms16;
   out->kernel_ms32 = s.kernel_ms32;
+  out->dp_bytes16 = s.dp_bytes16;
+  out->dp_bytes32 = s.dp_bytes32;
+  out->launches16 = s.launches16;
+  out->launches32 = s.launches32;
   return 0;
 }
 

@@ -769,8 +769,10 @@ int RunPassFinish(spoa_hip_engine* E) {
     if (gi < E->pass_width.size()) {
       if (E->pass_width[gi] == 16) {
         E->stats.kernel_ms16 += ms_k;
+        E->stats.launches16 += 1;
       } else {
         E->stats.kernel_ms32 += ms_k;
+        E->stats.launches32 += 1;
       }
     }
   }
@@ -890,8 +892,10 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     uint64_t R = 0;
     for (uint32_t r = 0; r < V; ++r) R += std::max<uint32_t>(1u, j.graph.pred_off[r + 1] - j.graph.pred_off[r]);
     E->stats.cells += static_cast<uint64_t>(V) * j.len;
-    E->stats.dp_bytes += static_cast<uint64_t>(width / 8) * j.len *
-                         (static_cast<uint64_t>(Matrices(P.subtype)) * V + static_cast<uint64_t>(ReadMatrices(P.subtype)) * R);
+    const uint64_t jb = static_cast<uint64_t>(width / 8) * j.len *
+                        (static_cast<uint64_t>(Matrices(P.subtype)) * V + static_cast<uint64_t>(ReadMatrices(P.subtype)) * R);
+    E->stats.dp_bytes += jb;
+    (width == 16 ? E->stats.dp_bytes16 : E->stats.dp_bytes32) += jb;
     E->stats.jobs += 1;
   }
   // group by (width, cfg); within a group the queue runs largest V * n first
