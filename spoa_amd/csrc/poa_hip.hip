@@ -708,13 +708,16 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
         return SPOA_HIP_DEVICE_ERROR;
       }
     }
-    if (beside && (!HipOk(hipEventRecord(E->ev_side, E->side), "event") ||
-                   !HipOk(hipStreamWaitEvent(cs, E->ev_side, 0), "stream wait"))) {
-      return SPOA_HIP_DEVICE_ERROR;
-    }
     E->pass_launches = groups.size();
+    // the copy out waits for both streams; the compute stream itself does
+    // not wait for the side stream, so the next pass's first launch (on a
+    // shared compute stream) can run beside this pass's side launches
     if (cs != st && (!HipOk(hipEventRecord(E->ev_kdone, cs), "event") ||
                      !HipOk(hipStreamWaitEvent(st, E->ev_kdone, 0), "stream wait"))) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+    if (beside && (!HipOk(hipEventRecord(E->ev_side, E->side), "event") ||
+                   !HipOk(hipStreamWaitEvent(st, E->ev_side, 0), "stream wait"))) {
       return SPOA_HIP_DEVICE_ERROR;
     }
     if (!HipOk(hipEventRecord(E->ev2, st), "event") ||
