@@ -1109,6 +1109,41 @@ PoaAlignKernel(const KArgs a) {
               fv[i] = vf[i];
               if (SUB == 2) ov[i] = vo[i];
             }
+            // the prefetch registers are free once the first predecessor is
+            // folded: the next prefetch leaves now, with the rest of this
+            // row's work to land before the drain ahead of its stores
+            // prefetch the first predecessor of the next (row, tile): the next
+            // tile of this row, or the first tile of the next row unless that
+            // predecessor is this row kept in registers.  Every predecessor
+            // tile was written earlier by these same lanes.
+            pf_valid = false;
+            {
+              bool want = false;
+              uint32_t q = 0, qc0 = 0;
+              if (t + 1 < ntiles) {
+                want = true;
+                q = p0;
+                qc0 = c0 + tile_w;
+              } else if (r < V && !(single && nxt.y == r)) {
+                want = true;
+                q = nxt.y;
+                qc0 = static_cast<uint32_t>(tid) * K;
+              }
+              if (want) {
+                const __amdgpu_buffer_rsrc_t rq = RowRsrc(row_ptr(q), rbytes);
+                const bool qa = qc0 < stride;
+                const uint32_t qoff = qc0 * S;
+                BufLoad<KW>(rq, qa ? qoff : rbytes, pfH);
+                if (SUB >= 1) BufLoad<KW>(rq, qa ? qoff + mbytes : rbytes, pfF);
+                if (SUB == 2) BufLoad<KW>(rq, qa ? qoff + 3 * mbytes : rbytes, pfO);
+                if (lane == 0 && qc0 > 0 && q > 0) {
+                  const uint32_t qt = t + 1 < ntiles ? t + 1 : 0u;
+                  pf_hm1 = bnd[static_cast<uint64_t>(qt * static_cast<uint32_t>(nwaves) + wave) * (V + 1) + q];
+                }
+                pf_valid = true;
+                pf_row = q;
+              }
+            }
           } else {
 #pragma unroll
             for (int i = 0; i < KW; ++i) {
@@ -1123,38 +1158,6 @@ PoaAlignKernel(const KArgs a) {
         for (int i = 0; i < KW; ++i) {
           h1[i] = WMax<ST>(h1[i], fv[i]);
           if (SUB == 2) h1[i] = WMax<ST>(h1[i], ov[i]);
-        }
-        // prefetch the first predecessor of the next (row, tile): the next
-        // tile of this row, or the first tile of the next row unless that
-        // predecessor is this row kept in registers.  Every predecessor
-        // tile was written earlier by these same lanes.
-        pf_valid = false;
-        {
-          bool want = false;
-          uint32_t q = 0, qc0 = 0;
-          if (t + 1 < ntiles) {
-            want = true;
-            q = p0;
-            qc0 = c0 + tile_w;
-          } else if (r < V && !(single && nxt.y == r)) {
-            want = true;
-            q = nxt.y;
-            qc0 = static_cast<uint32_t>(tid) * K;
-          }
-          if (want) {
-            const __amdgpu_buffer_rsrc_t rq = RowRsrc(row_ptr(q), rbytes);
-            const bool qa = qc0 < stride;
-            const uint32_t qoff = qc0 * S;
-            BufLoad<KW>(rq, qa ? qoff : rbytes, pfH);
-            if (SUB >= 1) BufLoad<KW>(rq, qa ? qoff + mbytes : rbytes, pfF);
-            if (SUB == 2) BufLoad<KW>(rq, qa ? qoff + 3 * mbytes : rbytes, pfO);
-            if (lane == 0 && qc0 > 0 && q > 0) {
-              const uint32_t qt = t + 1 < ntiles ? t + 1 : 0u;
-              pf_hm1 = bnd[static_cast<uint64_t>(qt * static_cast<uint32_t>(nwaves) + wave) * (V + 1) + q];
-            }
-            pf_valid = true;
-            pf_row = q;
-          }
         }
         if (single) {
 #pragma unroll
