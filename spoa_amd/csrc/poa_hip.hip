@@ -74,6 +74,30 @@ int WidthBits(const spoa_hip_params& p, uint32_t len, uint32_t V) {
 }
 
 int Lanes(int width_bits) { return width_bits == 16 ? 16 : 8; }
+
+// How one job runs: `width` is the reference's score width (what the caller
+// is told), `bits` the width of the stored rows, `lanes` the reference lane
+// count whose row width and convex segments the DP reproduces.  Local
+// alignments that the reference runs at int32 keep int16 rows when every
+// score fits them: H stays in [0, m * n] and every gap score is at least
+// its penalty above a non-negative H, so only m * (n + 8) needs headroom
+// (kept at 30000 below int16's maximum).
+struct Shape {
+  int width = 0, bits = 0, lanes = 0;
+};
+
+Shape JobShape(const spoa_hip_params& p, uint32_t len, uint32_t V) {
+  Shape sh;
+  sh.width = WidthBits(p, len, V);
+  if (sh.width == 0) return sh;
+  sh.lanes = Lanes(sh.width);
+  sh.bits = sh.width;
+  if (sh.width == 32 && p.type == SPOA_HIP_SW &&
+      static_cast<int64_t>(std::max<int32_t>(p.m, 0)) * (static_cast<int64_t>(len) + 8) <= 30000) {
+    sh.bits = 16;
+  }
+  return sh;
+}
 int Matrices(int subtype) { return subtype == 0 ? 1 : (subtype == 1 ? 3 : 5); }
 int ReadMatrices(int subtype) { return subtype == 0 ? 1 : (subtype == 1 ? 2 : 3); }
 
@@ -145,37 +169,40 @@ using KernelFn = void (*)(KArgs);
 // each lane 256 registers.
 constexpr int kMaxGroupWaves = 8;
 
-template <int SUB, typename ST, int K>
+template <int SUB, typename ST, int K, int L>
 KernelFn PickMaxT(int waves) {
-  if (waves == 1) return spoa_hip_dev::PoaAlignKernel<SUB, ST, K, 64>;
+  if (waves == 1) return spoa_hip_dev::PoaAlignKernel<SUB, ST, K, 64, L>;
   if constexpr (K == 32) {
     return nullptr;  // 32 columns a lane only in single-wave workgroups
   } else {
-    return spoa_hip_dev::PoaAlignKernel<SUB, ST, K, 64 * kMaxGroupWaves>;
+    return spoa_hip_dev::PoaAlignKernel<SUB, ST, K, 64 * kMaxGroupWaves, L>;
   }
 }
 
-template <int SUB, typename ST>
+template <int SUB, typename ST, int L>
 KernelFn PickK(int k, int waves) {
-  if (k == 32) return PickMaxT<SUB, ST, 32>(waves);
-  if (k == 16) return PickMaxT<SUB, ST, 16>(waves);
+  if (k == 32) return PickMaxT<SUB, ST, 32, L>(waves);
+  if (k == 16) return PickMaxT<SUB, ST, 16, L>(waves);
   if constexpr (SUB == 2 && sizeof(ST) == 2) {
-    return nullptr;  // convex int16 lanes must hold whole 16-column segments
+    return nullptr;  // convex int16 lanes hold whole 16-column segments (8-column lanes not built)
   } else {
-    return PickMaxT<SUB, ST, 8>(waves);
+    return PickMaxT<SUB, ST, 8, L>(waves);
   }
 }
 
-template <int SUB>
-KernelFn PickSub(int width_bits, int k, int waves) {
-  return width_bits == 16 ? PickK<SUB, int16_t>(k, waves) : PickK<SUB, int32_t>(k, waves);
-}
-
-KernelFn PickKernel(int subtype, int width_bits, int k, int waves) {
+// bits: stored rows; lanes: the reference lane count the launch reproduces.
+// Only convex rows depend on the lane count beyond the row width each job
+// carries, so linear and affine rows always run the kernel of their width's
+// own lane count; convex int16 rows with 8 lanes (lanes of 16 or 32
+// columns) hold the local alignments the reference scores at int32.
+KernelFn PickKernel(int subtype, int bits, int lanes, int k, int waves) {
   switch (subtype) {
-    case 0: return PickSub<0>(width_bits, k, waves);
-    case 1: return PickSub<1>(width_bits, k, waves);
-    default: return PickSub<2>(width_bits, k, waves);
+    case 0: return bits == 16 ? PickK<0, int16_t, 16>(k, waves) : PickK<0, int32_t, 8>(k, waves);
+    case 1: return bits == 16 ? PickK<1, int16_t, 16>(k, waves) : PickK<1, int32_t, 8>(k, waves);
+    default:
+      if (bits == 32) return PickK<2, int32_t, 8>(k, waves);
+      if (lanes == 8) return PickK<2, int16_t, 8>(k, waves);
+      return PickK<2, int16_t, 16>(k, waves);
   }
 }
 
@@ -189,7 +216,7 @@ void MpMul(const int32_t* a, const int32_t* b, int32_t* out) {
   std::memcpy(out, r, sizeof(r));
 }
 
-KParams MakeParams(const spoa_hip_params& p, int width_bits, int k) {
+KParams MakeParams(const spoa_hip_params& p, int width_bits, int lanes, int k) {
   KParams kp{};
   kp.type = p.type;
   kp.m = p.m;
@@ -206,7 +233,7 @@ KParams MakeParams(const spoa_hip_params& p, int width_bits, int k) {
   kp.decay = p.subtype == 0 ? k * p.g : k * p.e;
   if (p.subtype == 2) {
     // one L-column segment: (E, Q) entering the segment -> entering the next
-    const int L = Lanes(width_bits);
+    const int L = lanes;
     const int32_t seg[4] = {(L - 1) * p.e + std::max<int32_t>(p.g, p.e), (L - 1) * p.c + p.g,
                             (L - 1) * p.e + p.q, (L - 1) * p.c + std::max<int32_t>(p.q, p.c)};
     std::memcpy(kp.mseg, seg, sizeof(seg));
@@ -261,7 +288,9 @@ void FirstColumns(const spoa_hip_params& p, const spoa_hip_graph& gr, int32_t* f
 
 struct Plan {
   uint32_t job;
-  int width;
+  int width;  // the reference's score width
+  int bits;   // stored rows
+  int lanes;  // reference lane count (row width, convex segments)
   Cfg cfg;
   uint32_t W, stride;
   uint64_t mat_bytes;
@@ -426,11 +455,21 @@ void ParallelRanges(spoa_hip_engine* E, size_t n, const Fn& fn) {
 }
 
 KernelFn GroupKernel(spoa_hip_engine* E, const Plan& pl) {
-  return PickKernel(E->prm.subtype, pl.width, pl.cfg.k, pl.cfg.waves);
+  return PickKernel(E->prm.subtype, pl.bits, pl.lanes, pl.cfg.k, pl.cfg.waves);
 }
 
-bool SameGroup(const Plan& a, const Plan& b) {
-  return a.width == b.width && a.cfg.k == b.cfg.k && a.cfg.waves == b.cfg.waves;
+// Launch group of a job: jobs of one group share a kernel and its
+// parameters.  Linear and affine rows depend on the lane count only through
+// the row width each job carries, so their int16 rows share one group.
+int GroupLanes(int subtype, const Plan& pl) { return subtype == 2 || pl.bits == 32 ? pl.lanes : 16; }
+
+// Statistics bucket (16 / 32) of a job: the kernel it runs on (convex
+// int16 rows with 8-column segments count with the int32 launches).
+int Bucket(int subtype, const Plan& pl) { return pl.bits == 16 && GroupLanes(subtype, pl) == 16 ? 16 : 32; }
+
+bool SameGroup(int subtype, const Plan& a, const Plan& b) {
+  return a.bits == b.bits && GroupLanes(subtype, a) == GroupLanes(subtype, b) && a.cfg.k == b.cfg.k &&
+         a.cfg.waves == b.cfg.waves;
 }
 
 // One pass: the jobs plans[lo, hi) (sorted by launch group, largest first
@@ -560,7 +599,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   std::vector<Group> groups;
   for (size_t x = lo; x < hi;) {
     size_t y = x;
-    while (y < hi && SameGroup(plans[y], plans[x])) ++y;
+    while (y < hi && SameGroup(E->prm.subtype, plans[y], plans[x])) ++y;
     Group gp;
     gp.a = x - lo;
     gp.b = y - lo;
@@ -600,7 +639,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     }
     gp.grid = std::max<uint32_t>(
         1u, std::min<uint32_t>(n, static_cast<uint32_t>(std::max(1, per_cu)) * static_cast<uint32_t>(E->num_cus)));
-    gp.prm = MakeParams(P, plans[x].width, plans[x].cfg.k);
+    gp.prm = MakeParams(P, plans[x].bits, plans[x].lanes, plans[x].cfg.k);
     groups.push_back(gp);
     x = y;
   }
@@ -637,7 +676,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     return args;
   };
   E->pass_width.clear();
-  for (const Group& gp : groups) E->pass_width.push_back(plans[lo + gp.a].width);
+  for (const Group& gp : groups) E->pass_width.push_back(Bucket(E->prm.subtype, plans[lo + gp.a]));
   E->stats.launches += static_cast<uint32_t>(groups.size());
   E->graph_pass = E->graph_mode;
   if (E->graph_mode) {
@@ -671,7 +710,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
       return SPOA_HIP_DEVICE_ERROR;
     }
     E->pass_launches = 1;
-    E->pass_width.assign(1, plans[lo].width);
+    E->pass_width.assign(1, Bucket(E->prm.subtype, plans[lo]));
   } else {
     if (!HipOk(hipEventRecord(E->ev0, st), "event") ||
         !HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in") ||
@@ -851,8 +890,8 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
       if (j.graph.num_nodes == 0 || j.len == 0 || j.len > static_cast<uint32_t>(std::numeric_limits<int32_t>::max())) {
         continue;
       }
-      const int width = WidthBits(P, j.len, j.graph.num_nodes);
-      if (width != 0) cfg_len[width == 32] = std::max(cfg_len[width == 32], j.len);
+      const Shape sh = JobShape(P, j.len, j.graph.num_nodes);
+      if (sh.width != 0) cfg_len[sh.bits == 32] = std::max(cfg_len[sh.bits == 32], j.len);
     }
   }
   std::vector<Plan> plans;
@@ -870,19 +909,21 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     }
     const uint32_t V = j.graph.num_nodes;
     if (V == 0 || j.len == 0) continue;
-    const int width = WidthBits(P, j.len, V);
-    if (width == 0) {
+    const Shape sh = JobShape(P, j.len, V);
+    if (sh.width == 0) {
       fail(j, SPOA_HIP_OVERFLOW, kMsgOverflow);
       continue;
     }
     Plan pl;
     pl.job = x;
-    pl.width = width;
-    pl.cfg = ChooseCfg(E, batch ? cfg_len[width == 32] : j.len, width, batch);
-    const int L = Lanes(width);
+    pl.width = sh.width;
+    pl.bits = sh.bits;
+    pl.lanes = sh.lanes;
+    pl.cfg = ChooseCfg(E, batch ? cfg_len[sh.bits == 32] : j.len, sh.bits, batch);
+    const int L = sh.lanes;
     pl.W = (j.len + L - 1) / L * L;
     pl.stride = static_cast<uint32_t>(RoundUp(pl.W, static_cast<uint64_t>(pl.cfg.k)));
-    pl.mat_bytes = JobBytes(P, V, pl.stride, width, pl.cfg);
+    pl.mat_bytes = JobBytes(P, V, pl.stride, sh.bits, pl.cfg);
     const uint32_t E_in = j.graph.pred_off[V] - j.graph.pred_off[0];
     pl.graph_words = RoundUp(9ull * V + 4 + E_in, 4);
     pl.seq_bytes = RoundUp(pl.stride + 16, 16);
@@ -895,15 +936,17 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     uint64_t R = 0;
     for (uint32_t r = 0; r < V; ++r) R += std::max<uint32_t>(1u, j.graph.pred_off[r + 1] - j.graph.pred_off[r]);
     E->stats.cells += static_cast<uint64_t>(V) * j.len;
-    const uint64_t jb = static_cast<uint64_t>(width / 8) * j.len *
+    const uint64_t jb = static_cast<uint64_t>(sh.bits / 8) * j.len *
                         (static_cast<uint64_t>(Matrices(P.subtype)) * V + static_cast<uint64_t>(ReadMatrices(P.subtype)) * R);
     E->stats.dp_bytes += jb;
-    (width == 16 ? E->stats.dp_bytes16 : E->stats.dp_bytes32) += jb;
+    (Bucket(P.subtype, pl) == 16 ? E->stats.dp_bytes16 : E->stats.dp_bytes32) += jb;
     E->stats.jobs += 1;
   }
-  // group by (width, cfg); within a group the queue runs largest V * n first
+  // group by (row width, lanes, cfg); within a group the queue runs largest V * n first
   std::stable_sort(plans.begin(), plans.end(), [&](const Plan& a, const Plan& b) {
-    if (a.width != b.width) return a.width < b.width;
+    if (a.bits != b.bits) return a.bits < b.bits;
+    const int la = GroupLanes(P.subtype, a), lb = GroupLanes(P.subtype, b);
+    if (la != lb) return la > lb;
     if (a.cfg.k != b.cfg.k) return a.cfg.k < b.cfg.k;
     if (a.cfg.waves != b.cfg.waves) return a.cfg.waves < b.cfg.waves;
     const uint64_t wa = static_cast<uint64_t>(jobs[a.job].graph.num_nodes) * jobs[a.job].len;
@@ -1030,7 +1073,7 @@ int spoa_hip_prealloc(spoa_hip_engine* E, uint32_t max_len, uint32_t alphabet_si
   const int width = w < std::numeric_limits<int16_t>::min() + 1024 ? 32 : 16;
   if (!HipOk(hipSetDevice(E->device), "hipSetDevice")) return SPOA_HIP_DEVICE_ERROR;
   const Cfg cfg = ChooseCfg(E, max_len, width, false);
-  const int L = Lanes(width);
+  const int L = Lanes(width);  // the reference's sizing (int32 rows at most)
   const uint64_t W = (static_cast<uint64_t>(max_len) + L - 1) / L * L;
   const uint64_t stride = RoundUp(W, static_cast<uint64_t>(cfg.k));
   if (V >= static_cast<int64_t>(std::numeric_limits<uint32_t>::max())) return Fail(SPOA_HIP_NO_MEMORY, kMsgPreNoMemory);
@@ -1189,13 +1232,13 @@ uint64_t spoa_hip_memory_budget(spoa_hip_engine* E) { return E ? E->budget : 0; 
 
 uint64_t spoa_hip_job_footprint(spoa_hip_engine* E, uint32_t V, uint32_t len, int32_t batch) {
   if (!E || V == 0 || len == 0) return 0;
-  const int width = WidthBits(E->prm, len, V);
-  if (width == 0) return 0;
-  const Cfg cfg = ChooseCfg(E, len, width, batch != 0);
-  const int L = Lanes(width);
+  const Shape sh = JobShape(E->prm, len, V);
+  if (sh.width == 0) return 0;
+  const Cfg cfg = ChooseCfg(E, len, sh.bits, batch != 0);
+  const int L = sh.lanes;
   const uint32_t W = (len + L - 1) / L * L;
   const uint32_t stride = static_cast<uint32_t>(RoundUp(W, static_cast<uint64_t>(cfg.k)));
-  return JobBytes(E->prm, V, stride, width, cfg);
+  return JobBytes(E->prm, V, stride, sh.bits, cfg);
 }
 
 }  // extern "C"

@@ -821,10 +821,17 @@ struct WavesPerSimd {
   static constexpr int value = (sizeof(ST) == 2 ? K <= 16 : K <= 8) ? 3 : 1;
 };
 
-template <int SUB, typename ST, int K, int MAXT>
+// L: the reference engine's lane count, whose row width and convex E/Q
+// segments the launch reproduces -- 16 (its int16 engine) or 8 (its int32
+// engine).  Int16 rows with L = 8 hold convex local alignments for which
+// the reference picks int32 scores (its bound covers any alignment type)
+// although a local alignment's scores stay within int16: every H is in
+// [0, m * n] and every E, F, O, Q is at least its opening penalty above a
+// non-negative H (the host admits a job only when m * (n + 8) leaves int16
+// headroom).
+template <int SUB, typename ST, int K, int MAXT, int L>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<ST, K>::value)))
 PoaAlignKernel(const KArgs a) {
-  constexpr int L = sizeof(ST) == 2 ? 16 : 8;
   constexpr int CPW = Cols<ST>::kPerWord;
   constexpr int KW = K / CPW;  // score words a lane
   constexpr int PW = K / 2;    // profile words a lane

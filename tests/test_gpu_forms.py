@@ -9,6 +9,7 @@ import gzip
 import hashlib
 import json
 import os
+import random
 
 import pytest
 
@@ -105,6 +106,41 @@ def test_batched_matches_oracle_many_windows(cfg):
         cons, msa, _ = _oracle_window(c["params"], ws.reads(w))
         assert res.consensus[w] == cons, w
         assert res.msa[w] == msa, w
+
+
+def test_batched_local_int32_windows_beside_int16_ones():
+    """One batch of local convex windows whose later reads the reference
+    scores at int32 (graphs past its bound) next to reads it scores at
+    int16: the int32 ones run on int16 rows with 8-column E/Q segments, a
+    launch of their own beside the int16 one.  Every window equals the
+    oracle run alone, with the reference's widths."""
+    params = (spoa_amd.SW, 5, -4, -40, -25, -50, -12)
+    rng = random.Random(2024)
+    windows = []
+    for w in range(10):
+        templ = "".join(rng.choice("ACGT") for _ in range(rng.choice([250, 400])))
+        reads = []
+        for _ in range(rng.randint(8, 16)):
+            out = []
+            for b in templ:
+                x = rng.random()
+                if x < 0.08:
+                    out.append(rng.choice("ACGT"))
+                elif x < 0.16:
+                    out.append(b + rng.choice("ACGT"))
+                elif x >= 0.22:
+                    out.append(b)
+            reads.append("".join(out))
+        windows.append(reads)
+    ws = spoa_amd.Windows.from_lists(windows)
+    res = spoa_amd.run_windows(ws, *params, mode="batched", msa=True)
+    widths = set()
+    for w, reads in enumerate(windows):
+        cons, msa, recs = _oracle_window(params, reads)
+        widths.update(r["width"] for r in recs)
+        assert res.consensus[w] == cons, w
+        assert res.msa[w] == msa, w
+    assert {16, 32} <= widths  # (0: a read into the empty graph)
 
 
 def test_context_reuse_and_budget_split():

@@ -99,3 +99,26 @@ def test_random_windows(gpu, orc, seed):
             assert str(gex.value) == str(ex)
             continue
         _compare(gpu, orc, params, reads, strand=strand)
+
+
+# (n, g, e, q, c) per subtype: gap penalties large enough that the
+# reference's score bound picks int32 scores for graphs of a few hundred nodes
+_WIDE_GAPS = {0: (-4, -60, -60, -60, -60), 1: (-4, -100, -40, -100, -40), 2: (-4, -100, -40, -120, -20)}
+
+
+@pytest.mark.parametrize("sub", [0, 1, 2])
+def test_local_int32_scores_on_int16_rows(gpu, orc, sub):
+    """Local alignments that the reference scores at int32 although their
+    scores fit int16 run on int16 rows with the int32 engine's row width
+    (and, convex, its 8-column E/Q segments): the pairs, scores and reported
+    width of the reference's int32 engine.  With a match score that leaves
+    no int16 headroom (120 * (n + 8) > 30000) the jobs stay on int32 rows,
+    also equal."""
+    rng = random.Random(77 + sub)
+    for m in (5, 120):
+        params = (ol.SW, m) + _WIDE_GAPS[sub]
+        templ = "".join(rng.choice("ACGT") for _ in range(300))
+        reads = [_mutate(rng, templ, 0.12, "ACGT") for _ in range(6)]
+        _, _, recs = ol.run_window(orc, params, reads)
+        assert any(r["width"] == 32 for r in recs)
+        _compare(gpu, orc, params, reads)
