@@ -70,14 +70,13 @@ def dominant_kernel_roofline(ctx, stats, params, windows):
     n = sum(s["launches%d" % w] for s in stats)
     achieved = byts / (ms / 1e3) / 1e9 if ms > 0 else 0.0
     # the kernel of that width: the reference's int32 local convex jobs run
-    # on int16 rows with 8-column segments (the int16 launch configuration),
-    # other int32 jobs on int32 rows; launch configuration for the batch's
-    # longest read
+    # on int16 rows with 8-column segments, other int32 jobs on int32 rows;
+    # launch configuration of that width for the batch's longest read
     sub = spoa_amd.AlignmentEngine.create(*params).subtype
     narrow = w == 32 and sub == 2 and params[0] == spoa_amd.SW
     bits = 16 if w == 16 or narrow else 32
     k, waves = ctypes.c_int32(0), ctypes.c_int32(0)
-    spoa_amd.lib().spoa_hip_config(ctx.device, max(1, windows.max_read_len()), bits, 1, ctypes.byref(k),
+    spoa_amd.lib().spoa_hip_config(ctx.device, max(1, windows.max_read_len()), w, 1, ctypes.byref(k),
                                    ctypes.byref(waves))
     name = "PoaAlignKernel<%d, %s, %d, %d, %d>" % (sub, "short" if bits == 16 else "int", k.value,
                                                     64 if waves.value == 1 else 512, 16 if w == 16 else 8)

@@ -122,6 +122,7 @@ struct KArgs {
   int32_t* pairs;            // (node id, column) pairs in traceback order
   int32_t* results;          // per job: n_pairs, score, has_score, status
   uint32_t lds_bytes;        // dynamic LDS of the launch (at least the query profile's)
+  uint32_t prio_jobs;        // the first prio_jobs jobs of the queue run at raised wave priority
 };
 
 enum DevStatus : int32_t { kDevOk = 0, kDevCapacity = 6, kDevInconsistent = 7 };
@@ -881,6 +882,14 @@ PoaAlignKernel(const KArgs a) {
     const uint32_t jx = Uniform(s_job);
     __syncthreads();
     if (jx >= a.n_jobs) break;
+    // The launch lasts as long as its largest jobs (the queue hands them
+    // out first): their waves take the SIMDs' issue slots before those of
+    // the smaller jobs beside them, which have time to spare.
+    if (jx < a.prio_jobs) {
+      __builtin_amdgcn_s_setprio(2);
+    } else {
+      __builtin_amdgcn_s_setprio(0);
+    }
 
     // the job record, every field uniform (scalar registers)
     const DevJob Jv = a.jobs[jx];
