@@ -17,8 +17,6 @@ void PoaGraph::Clear() {
   in_.clear();
   out_.clear();
   aligned_.clear();
-  tail_.clear();
-  head_.clear();
   weight_.clear();
   path_.clear();
   path_off_.assign(1, 0);
@@ -43,18 +41,16 @@ std::uint32_t PoaGraph::AddNode(std::uint32_t code) {
 
 // src/graph.cpp:81-91: a tail->head edge that exists gains the weight
 void PoaGraph::AddEdge(std::uint32_t t, std::uint32_t h, std::uint32_t w) {
-  for (const std::uint32_t* e = out_.begin(t); e != out_.end(t); ++e) {
-    if (head_[*e] == h) {
-      weight_[*e] += w;
+  for (const Adj* e = out_.begin(t); e != out_.end(t); ++e) {
+    if (e->node == h) {
+      weight_[e->edge] += w;
       return;
     }
   }
-  const std::uint32_t e = static_cast<std::uint32_t>(tail_.size());
-  tail_.push_back(t);
-  head_.push_back(h);
+  const std::uint32_t e = static_cast<std::uint32_t>(weight_.size());
   weight_.push_back(w);
-  out_.push(t, e);
-  in_.push(h, e);
+  out_.push(t, Adj{h, e});
+  in_.push(h, Adj{t, e});
 }
 
 // src/graph.cpp:93-110: a fresh chain for seq[begin, end) (consecutive node
@@ -167,7 +163,7 @@ void PoaGraph::TopologicalSort() {
   v_sink_.resize(V);
   v_id_.resize(V);
   v_off_.resize(V + 1);
-  v_pred_.resize(tail_.size());
+  v_pred_.resize(weight_.size());
   std::uint32_t* const r2n = rank_to_node_.data();
   std::uint32_t* const rof = rank_of_.data();
   std::uint32_t* const off = v_off_.data();
@@ -176,14 +172,13 @@ void PoaGraph::TopologicalSort() {
   std::uint8_t* const vsk = v_sink_.data();
   std::int32_t* const vid = v_id_.data();
   std::uint8_t* const mk = marks_.data();
-  const std::uint32_t* const tl = tail_.data();
   const std::uint8_t* const cd = code_.data();
   std::uint32_t r = 0, p = 0;
   auto emit = [&](std::uint32_t id) {
     rof[id] = r;
     r2n[r] = id;
     off[r] = p;
-    for (const std::uint32_t* e = in_.begin(id); e != in_.end(id); ++e) pred[p++] = rof[tl[*e]];
+    for (const Adj* e = in_.begin(id); e != in_.end(id); ++e) pred[p++] = rof[e->node];
     vch[r] = static_cast<std::uint8_t>(decoder_[cd[id]]);
     vid[r] = static_cast<std::int32_t>(id);
     vsk[r] = out_.size(id) == 0 ? 1 : 0;
@@ -197,8 +192,8 @@ void PoaGraph::TopologicalSort() {
       const std::uint32_t curr = stack_.back();
       if ((mk[curr] & 2) == 0) {
         bool valid = true;
-        for (const std::uint32_t* e = in_.begin(curr); e != in_.end(curr); ++e) {
-          const std::uint32_t t = tl[*e];
+        for (const Adj* e = in_.begin(curr); e != in_.end(curr); ++e) {
+          const std::uint32_t t = e->node;
           if ((mk[t] & 2) == 0) {
             stack_.push_back(t);
             valid = false;
@@ -245,10 +240,9 @@ spoa_hip_graph PoaGraph::View() const {
 std::uint32_t PoaGraph::BranchCompletion(std::uint32_t rank, std::vector<std::int64_t>& score,
                                          std::vector<std::uint32_t>& pred) {
   const std::uint32_t start = rank_to_node_[rank];
-  for (const std::uint32_t* e = out_.begin(start); e != out_.end(start); ++e) {
-    const std::uint32_t hd = head_[*e];
-    for (const std::uint32_t* f = in_.begin(hd); f != in_.end(hd); ++f) {
-      if (tail_[*f] != start) score[tail_[*f]] = -1;
+  for (const Adj* e = out_.begin(start); e != out_.end(start); ++e) {
+    for (const Adj* f = in_.begin(e->node); f != in_.end(e->node); ++f) {
+      if (f->node != start) score[f->node] = -1;
     }
   }
   std::uint32_t best = kNil;
@@ -256,8 +250,8 @@ std::uint32_t PoaGraph::BranchCompletion(std::uint32_t rank, std::vector<std::in
     const std::uint32_t it = rank_to_node_[i];
     score[it] = -1;
     pred[it] = kNil;
-    for (const std::uint32_t* ep = in_.begin(it); ep != in_.end(it); ++ep) {
-      const std::uint32_t e = *ep, t = tail_[e];
+    for (const Adj* ep = in_.begin(it); ep != in_.end(it); ++ep) {
+      const std::uint32_t e = ep->edge, t = ep->node;
       if (score[t] == -1) continue;
       if (score[it] < weight_[e] || (score[it] == weight_[e] && score[pred[it]] <= score[t])) {
         score[it] = weight_[e];
@@ -277,8 +271,8 @@ void PoaGraph::TraverseHeaviestBundle() {
   std::vector<std::int64_t> score(num_nodes(), -1);
   std::uint32_t best = kNil;
   for (std::uint32_t it : rank_to_node_) {
-    for (const std::uint32_t* ep = in_.begin(it); ep != in_.end(it); ++ep) {
-      const std::uint32_t e = *ep, t = tail_[e];
+    for (const Adj* ep = in_.begin(it); ep != in_.end(it); ++ep) {
+      const std::uint32_t e = ep->edge, t = ep->node;
       if (score[it] < weight_[e] || (score[it] == weight_[e] && score[pred[it]] <= score[t])) {
         score[it] = weight_[e];
         pred[it] = t;

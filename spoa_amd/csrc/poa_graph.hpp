@@ -26,9 +26,10 @@
 
 namespace spoa_amd {
 
-// Per-owner lists of uint32 in one pool: a list is a block [start, start +
+// Per-owner lists of T in one pool: a list is a block [start, start +
 // size) of capacity cap; a full list moves to a block of twice the capacity
 // at the end of the pool (or grows in place when it is the last block).
+template <typename T>
 class BlockLists {
  public:
   void clear() {
@@ -37,10 +38,10 @@ class BlockLists {
   }
   void add_owner() { meta_.push_back(Meta{0, 0, 0}); }
   std::uint32_t size(std::uint32_t o) const { return meta_[o].size; }
-  const std::uint32_t* begin(std::uint32_t o) const { return pool_.data() + meta_[o].start; }
-  const std::uint32_t* end(std::uint32_t o) const { return pool_.data() + meta_[o].start + meta_[o].size; }
-  std::uint32_t at(std::uint32_t o, std::uint32_t i) const { return pool_[meta_[o].start + i]; }
-  void push(std::uint32_t o, std::uint32_t v) {
+  const T* begin(std::uint32_t o) const { return pool_.data() + meta_[o].start; }
+  const T* end(std::uint32_t o) const { return pool_.data() + meta_[o].start + meta_[o].size; }
+  T at(std::uint32_t o, std::uint32_t i) const { return pool_[meta_[o].start + i]; }
+  void push(std::uint32_t o, T v) {
     Meta& m = meta_[o];
     if (m.size == m.cap) Grow(m);
     pool_[m.start + m.size++] = v;
@@ -62,8 +63,15 @@ class BlockLists {
     }
     m.cap = cap;
   }
-  std::vector<std::uint32_t> pool_;
+  std::vector<T> pool_;
   std::vector<Meta> meta_;
+};
+
+// One entry of a node's in-edge (out-edge) list: the edge's other end --
+// its tail (head) -- kept beside the edge id, so walks over the lists read
+// the neighbour without a lookup by edge.
+struct Adj {
+  std::uint32_t node, edge;
 };
 
 class PoaGraph {
@@ -108,11 +116,11 @@ class PoaGraph {
   std::int32_t decoder_[256];
   std::uint32_t num_codes_ = 0;
 
-  // nodes: code, in-edges, out-edges (edge ids), aligned nodes (node ids)
+  // nodes: code, in-edges, out-edges, aligned nodes (node ids)
   std::vector<std::uint8_t> code_;
-  BlockLists in_, out_, aligned_;
-  // edges
-  std::vector<std::uint32_t> tail_, head_;
+  BlockLists<Adj> in_, out_;
+  BlockLists<std::uint32_t> aligned_;
+  // edges (their ends live in the in- and out-edge lists)
   std::vector<std::int64_t> weight_;
   // sequences: sequence s visits path_[path_off_[s] .. path_off_[s + 1])
   std::vector<std::uint32_t> path_, path_off_;
