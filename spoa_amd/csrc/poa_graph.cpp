@@ -190,7 +190,17 @@ void PoaGraph::TopologicalSort() {
     stack_.push_back(root);
     while (!stack_.empty()) {
       const std::uint32_t curr = stack_.back();
-      if ((mk[curr] & 2) == 0) {
+      if ((mk[curr] & 3) == 1) {
+        // Back on top after its first visit: every node that visit pushed
+        // (its unfinished in-edge tails and aligned nodes) has been finished
+        // and popped above it, so the reference's second scan of its lists
+        // would find them all finished.
+        mk[curr] |= 2;
+        if ((mk[curr] & 4) == 0) {
+          emit(curr);
+          for (const std::uint32_t* a = aligned_.begin(curr); a != aligned_.end(curr); ++a) emit(*a);
+        }
+      } else if ((mk[curr] & 2) == 0) {
         bool valid = true;
         for (const Adj* e = in_.begin(curr); e != in_.end(curr); ++e) {
           const std::uint32_t t = e->node;
