@@ -309,14 +309,15 @@ struct spoa_hip_engine {
   hipStream_t compute = nullptr;
   hipStream_t compute_own = nullptr;  // created when other engines share this one's kernels
   // launch groups after a pass's first run on this stream, beside the first
-  // (for example the few int32 jobs of a pass beside its int16 jobs)
-  // (made on first use: streams map onto the device's few hardware queues
-  // in creation order, and engines that never need one must not shift
-  // that mapping for the streams made after them)
+  // (for example the few int32-width jobs of a pass beside its int16 jobs):
+  // the engine's own stream when its kernels run on a shared compute stream
+  // (its copies are idle while its pass runs), else a stream made on first
+  // use (streams map onto the device's few hardware queues in creation
+  // order, and engines that never need one must not shift that mapping for
+  // the streams made after them)
   hipStream_t side = nullptr;
   hipStream_t side_own = nullptr;
-  spoa_hip_engine* side_owner = nullptr;  // whose side stream this engine shares (null: its own)
-  hipEvent_t ev_reset = nullptr, ev_side = nullptr;
+  hipEvent_t ev_side_in = nullptr, ev_side = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;  // copy timing
   hipEvent_t ev_in = nullptr, ev_kdone = nullptr;  // copy -> kernels -> copy ordering
   std::vector<hipEvent_t> ev_ks, ev_ke;            // start / end of every launch of a pass
@@ -738,22 +739,27 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
                      !HipOk(hipStreamWaitEvent(cs, E->ev_in, 0), "stream wait"))) {
       return SPOA_HIP_DEVICE_ERROR;
     }
-    spoa_hip_dev::ResetCounters<<<1, 256, 0, cs>>>(counters, static_cast<uint32_t>(groups.size()));
-    if (!HipOk(hipGetLastError(), "counter reset")) return SPOA_HIP_DEVICE_ERROR;
     // The first group runs on the compute stream, the others beside it on
     // the side stream (one after another there), each launch timed by its
-    // own events; the copy out waits for both streams.
+    // own events; the copy out waits for both streams.  Each stream zeroes
+    // the job counters of its own launches, so the side launches wait only
+    // for the copy in, not for the compute stream's earlier work.
     if (groups.size() > 1 && !E->side) {
-      spoa_hip_engine* so = E->side_owner ? E->side_owner : E;
-      if (!so->side_own && !HipOk(hipStreamCreateWithFlags(&so->side_own, hipStreamNonBlocking), "hipStreamCreate")) {
+      if (!E->side_own && !HipOk(hipStreamCreateWithFlags(&E->side_own, hipStreamNonBlocking), "hipStreamCreate")) {
         return SPOA_HIP_DEVICE_ERROR;
       }
-      E->side = so->side_own;
+      E->side = E->side_own;
     }
     const bool beside = groups.size() > 1 && E->side != cs;
-    if (beside && (!HipOk(hipEventRecord(E->ev_reset, cs), "event") ||
-                   !HipOk(hipStreamWaitEvent(E->side, E->ev_reset, 0), "stream wait"))) {
-      return SPOA_HIP_DEVICE_ERROR;
+    spoa_hip_dev::ResetCounters<<<1, 256, 0, cs>>>(counters, beside ? 1u : static_cast<uint32_t>(groups.size()));
+    if (!HipOk(hipGetLastError(), "counter reset")) return SPOA_HIP_DEVICE_ERROR;
+    if (beside) {
+      if (E->side != st && (!HipOk(hipEventRecord(E->ev_side_in, st), "event") ||
+                            !HipOk(hipStreamWaitEvent(E->side, E->ev_side_in, 0), "stream wait"))) {
+        return SPOA_HIP_DEVICE_ERROR;
+      }
+      spoa_hip_dev::ResetCounters<<<1, 256, 0, E->side>>>(counters + 1, static_cast<uint32_t>(groups.size() - 1));
+      if (!HipOk(hipGetLastError(), "counter reset")) return SPOA_HIP_DEVICE_ERROR;
     }
     for (size_t gi = 0; gi < groups.size(); ++gi) {
       const Group& gp = groups[gi];
@@ -1033,7 +1039,7 @@ int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_e
       !HipOk(hipEventCreate(&E->ev2), "hipEventCreate") || !HipOk(hipEventCreate(&E->ev3), "hipEventCreate") ||
       !HipOk(hipEventCreateWithFlags(&E->ev_in, hipEventDisableTiming), "hipEventCreate") ||
       !HipOk(hipEventCreateWithFlags(&E->ev_kdone, hipEventDisableTiming), "hipEventCreate") ||
-      !HipOk(hipEventCreateWithFlags(&E->ev_reset, hipEventDisableTiming), "hipEventCreate") ||
+      !HipOk(hipEventCreateWithFlags(&E->ev_side_in, hipEventDisableTiming), "hipEventCreate") ||
       !HipOk(hipEventCreateWithFlags(&E->ev_side, hipEventDisableTiming), "hipEventCreate")) {
     spoa_hip_engine_destroy(E);
     return SPOA_HIP_DEVICE_ERROR;
@@ -1064,7 +1070,7 @@ int spoa_hip_engine_destroy(spoa_hip_engine* E) {
   if (E->graph) (void)hipGraphDestroy(E->graph);
   if (E->ev_in) (void)hipEventDestroy(E->ev_in);
   if (E->ev_kdone) (void)hipEventDestroy(E->ev_kdone);
-  if (E->ev_reset) (void)hipEventDestroy(E->ev_reset);
+  if (E->ev_side_in) (void)hipEventDestroy(E->ev_side_in);
   if (E->ev_side) (void)hipEventDestroy(E->ev_side);
   if (E->side_own) {
     (void)hipStreamSynchronize(E->side_own);
@@ -1217,8 +1223,10 @@ int spoa_hip_share_compute(spoa_hip_engine* E, spoa_hip_engine* owner) {
     owner->compute = owner->compute_own;
   }
   E->compute = owner ? owner->compute : E->stream;
-  E->side_owner = owner;
-  E->side = nullptr;
+  // side launches on the engine's own stream (idle but for its copies), so
+  // the engines sharing the compute stream each have their own
+  if (owner) owner->side = owner->compute != owner->stream ? owner->stream : nullptr;
+  E->side = E->compute != E->stream ? E->stream : nullptr;
   return SPOA_HIP_OK;
 }
 
@@ -1230,7 +1238,6 @@ int spoa_hip_share_stream(spoa_hip_engine* E, spoa_hip_engine* owner) {
   if (E->pending.active) (void)RunPassFinish(E);
   E->compute = E->copy = owner ? owner->stream : E->stream;
   // everything of this engine on the owner's one stream
-  E->side_owner = nullptr;
   E->side = owner ? owner->stream : nullptr;
   return SPOA_HIP_OK;
 }
