@@ -596,7 +596,6 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     KParams prm;
   };
   std::vector<Group> groups;
-  const bool multi_group = !SameGroup(E->prm.subtype, plans[lo], plans[hi - 1]);  // plans sorted by group
   for (size_t x = lo; x < hi;) {
     size_t y = x;
     while (y < hi && SameGroup(E->prm.subtype, plans[y], plans[x])) ++y;
@@ -621,25 +620,11 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     int per_cu = 0;
     if (!blocks_per_cu(gp.lds, &per_cu)) return SPOA_HIP_DEVICE_ERROR;
     const uint32_t n = static_cast<uint32_t>(gp.b - gp.a);
-    // When the chip holds every job of the group at once, each workgroup
-    // may take more LDS (the traceback's tile grows with it) as long as
-    // they all still fit -- only in a pass of one launch group: launches of
-    // several groups share the device, and workgroups sized to fill it
-    // alone keep the others' workgroups waiting (8% more windows/s on
-    // 2,048 local convex 1.5 kb windows without the growth there).
-    const uint32_t need = (n + static_cast<uint32_t>(E->num_cus) - 1) / static_cast<uint32_t>(E->num_cus);
-    if (!multi_group && need <= static_cast<uint32_t>(per_cu)) {
-      for (uint32_t cand : {65536u, 49152u, 40960u, 32768u, 24576u}) {
-        if (cand <= gp.lds) break;
-        int pc = 0;
-        if (!blocks_per_cu(cand, &pc)) return SPOA_HIP_DEVICE_ERROR;
-        if (static_cast<uint32_t>(pc) >= need) {
-          gp.lds = cand;
-          per_cu = pc;
-          break;
-        }
-      }
-    }
+    // Workgroups take only the LDS of their query profile (the traceback's
+    // tile lives in it): launches of the two halves and of several launch
+    // groups share the device, and workgroups enlarged to fill it alone
+    // kept the other launches' workgroups waiting (11% more windows/s on
+    // 2,048 local convex 1.5 kb windows without the enlargement).
     gp.grid = std::max<uint32_t>(
         1u, std::min<uint32_t>(n, static_cast<uint32_t>(std::max(1, per_cu)) * static_cast<uint32_t>(E->num_cus)));
     gp.prm = MakeParams(P, plans[x].bits, plans[x].lanes, plans[x].cfg.k);
