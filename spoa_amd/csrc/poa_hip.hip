@@ -582,7 +582,29 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
               &d.nslots, &d.slot_chars);
     }
   };
-  ParallelRanges(E, nj, pack_range);
+  // Large passes on the stream path are packed in slices of jobs, each
+  // slice's three regions (job records, graph blocks, reads) copied in as
+  // soon as it is packed, so most of the transfer runs while the later
+  // slices are packed (a captured pass copies everything in one transfer).
+  const size_t nslices = !E->graph_mode && nj >= 256 ? 4 : 1;
+  if (nslices > 1 && !HipOk(hipEventRecord(E->ev0, E->copy), "event")) return SPOA_HIP_DEVICE_ERROR;
+  for (size_t sl = 0; sl < nslices; ++sl) {
+    const size_t a = nj * sl / nslices, b = nj * (sl + 1) / nslices;
+    ParallelRanges(E, b - a, [&](size_t u, size_t v) { pack_range(a + u, a + v); });
+    if (nslices == 1) break;
+    const uint64_t g0 = g_at[a] * 4, g1 = (b < nj ? g_at[b] : graph_words) * 4;
+    const uint64_t s0 = s_at[a], s1 = b < nj ? s_at[b] : seq_bytes;
+    uint8_t* din0 = static_cast<uint8_t*>(E->d_in.p);
+    const uint8_t* hin0 = E->h_in.At<uint8_t>(0);
+    auto copy = [&](uint64_t off, uint64_t bytes) {
+      return bytes == 0 ||
+             HipOk(hipMemcpyAsync(din0 + off, hin0 + off, bytes, hipMemcpyHostToDevice, E->copy), "copy in");
+    };
+    if (!copy(off_jobs + a * sizeof(DevJob), (b - a) * sizeof(DevJob)) || !copy(off_graph + g0, g1 - g0) ||
+        !copy(off_seq + s0, s1 - s0)) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+  }
   E->stats.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
 
   // ---- launch groups
@@ -715,11 +737,12 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     E->pass_launches = 1;
     E->pass_width.assign(1, plans[lo].width);
   } else {
-    if (!HipOk(hipEventRecord(E->ev0, st), "event") ||
-        !HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in") ||
-        !HipOk(hipEventRecord(E->ev1, st), "event")) {
+    if (nslices == 1 &&
+        (!HipOk(hipEventRecord(E->ev0, st), "event") ||
+         !HipOk(hipMemcpyAsync(E->d_in.p, E->h_in.p, in_bytes, hipMemcpyHostToDevice, st), "copy in"))) {
       return SPOA_HIP_DEVICE_ERROR;
     }
+    if (!HipOk(hipEventRecord(E->ev1, st), "event")) return SPOA_HIP_DEVICE_ERROR;
     if (cs != st && (!HipOk(hipEventRecord(E->ev_in, st), "event") ||
                      !HipOk(hipStreamWaitEvent(cs, E->ev_in, 0), "stream wait"))) {
       return SPOA_HIP_DEVICE_ERROR;

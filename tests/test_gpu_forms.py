@@ -143,6 +143,35 @@ def test_batched_local_int32_windows_beside_int16_ones():
     assert {16, 32} <= widths  # (0: a read into the empty graph)
 
 
+def test_batched_large_passes_match_oracle():
+    """Passes of hundreds of jobs (packed and copied in in slices): every
+    window of a 600-window batch equals the oracle run alone."""
+    params = (spoa_amd.SW, 5, -4, -8, -6, -10, -4)
+    rng = random.Random(606)
+    windows = []
+    for _ in range(600):
+        templ = "".join(rng.choice("ACGT") for _ in range(rng.randint(60, 160)))
+        reads = []
+        for _ in range(rng.randint(2, 4)):
+            out = []
+            for ch in templ:
+                x = rng.random()
+                if x < 0.05:
+                    out.append(rng.choice("ACGT"))
+                elif x < 0.10:
+                    out.append(ch + rng.choice("ACGT"))
+                elif x >= 0.14:
+                    out.append(ch)
+            reads.append("".join(out) or templ)
+        windows.append(reads)
+    ws = spoa_amd.Windows.from_lists(windows)
+    res = spoa_amd.run_windows(ws, *params, mode="batched", msa=True)
+    for w, reads in enumerate(windows):
+        cons, msa, _ = _oracle_window(params, reads)
+        assert res.consensus[w] == cons, w
+        assert res.msa[w] == msa, w
+
+
 def test_context_reuse_and_budget_split():
     """A reused batched context gives identical results run after run, also
     when a small memory budget splits every round into several passes."""
