@@ -144,8 +144,8 @@ def test_batched_local_int32_windows_beside_int16_ones():
 
 
 def test_batched_large_passes_match_oracle():
-    """Passes of hundreds of jobs (packed and copied in in slices): every
-    window of a 600-window batch equals the oracle run alone."""
+    """Passes of hundreds of jobs: every window of a 600-window batch equals
+    the oracle run alone."""
     params = (spoa_amd.SW, 5, -4, -8, -6, -10, -4)
     rng = random.Random(606)
     windows = []
@@ -168,6 +168,20 @@ def test_batched_large_passes_match_oracle():
     res = spoa_amd.run_windows(ws, *params, mode="batched", msa=True)
     for w, reads in enumerate(windows):
         cons, msa, _ = _oracle_window(params, reads)
+        assert res.consensus[w] == cons, w
+        assert res.msa[w] == msa, w
+
+
+def test_batched_full_size_batch_matches_oracle():
+    """A full 2,048-window batch of 1.5 kb local convex windows (passes of
+    1,024 jobs whose staging blocks pass 128 MiB in the later rounds, so
+    they are packed and copied in in slices): windows spread over the
+    batch equal the oracle run alone."""
+    c = spoa_amd.CONFIGS[2]
+    ws = spoa_amd.config_windows(2, seed=31, n_windows=2048)
+    res = spoa_amd.run_windows(ws, *c["params"], mode="batched", msa=True)
+    for w in (0, 700, 1401, 2047):
+        cons, msa, _ = _oracle_window(c["params"], ws.reads(w))
         assert res.consensus[w] == cons, w
         assert res.msa[w] == msa, w
 
