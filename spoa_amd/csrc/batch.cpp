@@ -95,6 +95,21 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& op
 
 namespace {
 
+// Grow-only int32 buffer that neither initialises nor copies its contents
+// (a job's pair buffer grows with its graph every round; a vector would
+// copy and zero-fill it each time).
+struct PairBuf {
+  std::unique_ptr<std::int32_t[]> p;
+  std::size_t cap = 0;
+  std::int32_t* Get(std::size_t n) {
+    if (n > cap) {
+      cap = n + n / 2;
+      p.reset(new std::int32_t[cap]);
+    }
+    return p.get();
+  }
+};
+
 // One half of the windows and its engine: the jobs of the current round
 // and where their results go.
 struct Half {
@@ -103,7 +118,7 @@ struct Half {
   std::vector<std::uint32_t> live;     // windows with a read in the current round
   std::vector<spoa_hip_job> jobs;
   std::vector<std::int32_t> job_of;    // per (live index, strand): job index or -1
-  std::vector<std::vector<std::int32_t>> pair_buf;
+  std::vector<PairBuf> pair_buf;
   std::uint32_t round = 0;
   bool pending = false;
 };
@@ -186,9 +201,7 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
         j.seq = d == 0 ? s : rc_seq[w].data();
         j.len = len;
         const std::uint32_t cap = j.graph.num_nodes + len;
-        auto& buf = hf.pair_buf[x * per_read + d];
-        buf.resize(2ull * cap);
-        j.pairs = buf.data();
+        j.pairs = hf.pair_buf[x * per_read + d].Get(2ull * cap);
         j.cap = cap;
         hf.job_of[x * per_read + d] = static_cast<std::int32_t>(hf.jobs.size());
         hf.jobs.push_back(j);

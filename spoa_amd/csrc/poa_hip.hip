@@ -397,8 +397,10 @@ uint64_t JobBytes(const spoa_hip_params& p, uint32_t V, uint32_t stride, int wid
 }
 
 // Packs one job's graph block and read into the staging buffer.
+// *r_out: R = sum over the rows of max(1, predecessors), for the DP's
+// algorithmic bytes.
 void PackJob(const spoa_hip_params& P, const spoa_hip_job& j, const Plan& pl, uint32_t* g, uint8_t* s,
-             uint32_t* nslots_out, uint32_t* slot_chars_out) {
+             uint32_t* nslots_out, uint32_t* slot_chars_out, uint64_t* r_out) {
   const spoa_hip_graph& gr = j.graph;
   const uint32_t V = gr.num_nodes;
   uint32_t* rows = g;
@@ -424,9 +426,11 @@ void PackJob(const spoa_hip_params& P, const spoa_hip_job& j, const Plan& pl, ui
   if (nslots > 4) nslots = 0;
   *nslots_out = nslots;
   *slot_chars_out = slot_chars;
+  uint64_t R = 0;
   for (uint32_t r = 0; r < V; ++r) {
     const uint32_t pb = gr.pred_off[r] - base, pe = gr.pred_off[r + 1] - base;
     const uint32_t npred = pe > pb ? pe - pb : 1u;
+    R += npred;
     const uint32_t slot = nslots ? static_cast<uint32_t>(slot_of[gr.row_char[r]]) : 7u;
     node_id[r] = gr.node_id[r];
     poff[r] = pb;
@@ -438,6 +442,7 @@ void PackJob(const spoa_hip_params& P, const spoa_hip_job& j, const Plan& pl, ui
   }
   poff[V] = gr.pred_off[V] - base;
   for (uint32_t k = base; k < gr.pred_off[V]; ++k) preds[k - base] = gr.pred_rank[k] + 1;
+  *r_out = R;
   std::memcpy(s, j.seq, j.len);
   std::memset(s + j.len, 0, pl.seq_bytes - j.len);
 }
@@ -564,6 +569,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   // ---- pack (host threads for large passes)
   const auto t_host0 = std::chrono::steady_clock::now();
   DevJob* dj = E->h_in.At<DevJob>(off_jobs);
+  std::vector<uint64_t> job_r(nj);
   auto pack_range = [&](size_t a, size_t b) {
     for (size_t x = a; x < b; ++x) {
       const Plan& pl = plans[lo + x];
@@ -579,7 +585,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
       d.pair_off = static_cast<uint32_t>(p_at[x]);
       d.cap = j.graph.num_nodes + j.len;
       PackJob(P, j, pl, E->h_in.At<uint32_t>(off_graph) + g_at[x], E->h_in.At<uint8_t>(off_seq) + s_at[x],
-              &d.nslots, &d.slot_chars);
+              &d.nslots, &d.slot_chars, &job_r[x]);
     }
   };
   // Passes of 128 MiB or more on the stream path are packed in slices of
@@ -606,6 +612,16 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
         !copy(off_seq + s0, s1 - s0)) {
       return SPOA_HIP_DEVICE_ERROR;
     }
+  }
+  // algorithmic DP bytes of the pass's jobs: S * n * (M * V + Rd * R)
+  for (size_t x = 0; x < nj; ++x) {
+    const Plan& pl = plans[lo + x];
+    const spoa_hip_job& j = jobs[pl.job];
+    const uint64_t jb = static_cast<uint64_t>(pl.bits / 8) * j.len *
+                        (static_cast<uint64_t>(Matrices(P.subtype)) * j.graph.num_nodes +
+                         static_cast<uint64_t>(ReadMatrices(P.subtype)) * job_r[x]);
+    E->stats.dp_bytes += jb;
+    (pl.width == 16 ? E->stats.dp_bytes16 : E->stats.dp_bytes32) += jb;
   }
   E->stats.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
 
@@ -968,14 +984,8 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
       continue;
     }
     plans.push_back(pl);
-    // algorithmic work (cells, bytes) of this DP
-    uint64_t R = 0;
-    for (uint32_t r = 0; r < V; ++r) R += std::max<uint32_t>(1u, j.graph.pred_off[r + 1] - j.graph.pred_off[r]);
+    // algorithmic work of this DP (its bytes are counted as the job is packed)
     E->stats.cells += static_cast<uint64_t>(V) * j.len;
-    const uint64_t jb = static_cast<uint64_t>(sh.bits / 8) * j.len *
-                        (static_cast<uint64_t>(Matrices(P.subtype)) * V + static_cast<uint64_t>(ReadMatrices(P.subtype)) * R);
-    E->stats.dp_bytes += jb;
-    (pl.width == 16 ? E->stats.dp_bytes16 : E->stats.dp_bytes32) += jb;
     E->stats.jobs += 1;
   }
   // group by (row width, lanes, cfg); within a group the queue runs largest V * n first
