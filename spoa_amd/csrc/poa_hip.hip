@@ -248,8 +248,17 @@ KParams MakeParams(const spoa_hip_params& p, int width_bits, int lanes, int k) {
 
 // First columns (reference simd...:568-577, 590-599, 607-694): fcF / fcO
 // are the vertical-gap columns, fcH the primary column per alignment type.
+// Local and overlap alignments start every row at 0 and never walk the
+// first column back (the traceback stops at column 0), so there only the
+// zero H column matters and the vertical-gap columns stay zero.
 void FirstColumns(const spoa_hip_params& p, const spoa_hip_graph& gr, int32_t* fcH, int32_t* fcF, int32_t* fcO) {
   const uint32_t V = gr.num_nodes;
+  if (p.type != SPOA_HIP_NW) {
+    std::memset(fcH, 0, (V + 1) * sizeof(int32_t));
+    std::memset(fcF, 0, (V + 1) * sizeof(int32_t));
+    std::memset(fcO, 0, (V + 1) * sizeof(int32_t));
+    return;
+  }
   const int32_t kNinf = std::numeric_limits<int32_t>::min() + 1024;
   fcF[0] = 0;
   fcO[0] = 0;
