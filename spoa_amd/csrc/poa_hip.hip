@@ -669,11 +669,23 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     int per_cu = 0;
     if (!blocks_per_cu(gp.lds, &per_cu)) return SPOA_HIP_DEVICE_ERROR;
     const uint32_t n = static_cast<uint32_t>(gp.b - gp.a);
-    // Workgroups take only the LDS of their query profile (the traceback's
-    // tile lives in it): launches of the two halves and of several launch
-    // groups share the device, and workgroups enlarged to fill it alone
-    // kept the other launches' workgroups waiting (11% more windows/s on
-    // 2,048 local convex 1.5 kb windows without the enlargement).
+    // Workgroups take the LDS of their query profile (the traceback's tile
+    // lives in it), enlarged to 24 or 32 KiB only where that leaves the
+    // kernel's own workgroups per CU (set by its registers) unchanged:
+    // launches of the two halves and of several launch groups share the
+    // device, and workgroups enlarged to fill it alone kept the other
+    // launches' workgroups waiting (11% more windows/s on 2,048 local
+    // convex 1.5 kb windows without that enlargement; about 1% more with
+    // this one).
+    for (uint32_t cand : {32768u, 24576u}) {
+      if (cand <= gp.lds) break;
+      int pc = 0;
+      if (!blocks_per_cu(cand, &pc)) return SPOA_HIP_DEVICE_ERROR;
+      if (pc >= per_cu) {
+        gp.lds = cand;
+        break;
+      }
+    }
     gp.grid = std::max<uint32_t>(
         1u, std::min<uint32_t>(n, static_cast<uint32_t>(std::max(1, per_cu)) * static_cast<uint32_t>(E->num_cus)));
     gp.prm = MakeParams(P, plans[x].bits, plans[x].lanes, plans[x].cfg.k);
