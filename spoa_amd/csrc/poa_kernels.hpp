@@ -3,8 +3,10 @@
 //
 // Semantics follow the reference SIMD engine
 // (src/simd_alignment_engine_implementation.hpp of spoa v4.0.8, AVX2 lane
-// counts): rows of W = ceil(n / L) * L columns with L = 16 (int16 scores)
-// or L = 8 (int32 scores), padding columns scored with
+// counts): rows of W = ceil(n / L) * L columns with L = 16 (its int16
+// engine) or L = 8 (its int32 engine; the kernels take L as a template
+// parameter, so int16 rows can reproduce the int32 engine), padding
+// columns scored with
 // -max(|m|, |n|, |g|, |q|) (:526-542), linear / affine / convex
 // recurrences (:824-901, :1179-1272, :1638-1774), the end-cell choice
 // (:903-953) and the traceback with its tie order (:955-1124, :1324-1574,
