@@ -367,9 +367,9 @@ namespace {
 
 // Batched launches: 16 columns a lane for int16 rows (at most 1,024
 // columns a wave), 8 for int32 rows (512 a wave), as many waves as the row
-// needs up to 8 (wider rows take several tiles).  Measured on cfg-2/cfg-3
-// shaped batches, this beats both longer lanes (K = 32, one wave) and more
-// waves of shorter lanes.  The per-call launch takes the widest workgroup
+// needs up to 8 (wider rows take several tiles).  Measured on batches of
+// 1.5 kb local convex and 500 bp overlap affine windows, this beats both
+// longer lanes (K = 32, one wave) and more waves of shorter lanes.  The per-call launch takes the widest workgroup
 // (up to 8 waves) of the shortest lanes.
 Cfg DefaultCfg(const spoa_hip_engine* E, uint32_t W, int width_bits, bool batch) {
   Cfg c;
