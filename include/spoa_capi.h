@@ -127,6 +127,13 @@ int64_t spoa_batch_consensus_all(void* result, char* out, uint64_t cap, uint64_t
 int spoa_batch_msa(void* result, uint32_t window, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols);
 int spoa_batch_reversed(void* result, uint32_t window, uint8_t* out, uint32_t cap);
 
+/* The reverse complement the strand-ambiguous mode aligns (reference
+ * src/main.cpp:286, 296, biosoup Sequence::ReverseAndComplement, not
+ * vendored by the reference): upper-case, then the Watson-Crick / IUPAC
+ * complement (A-T, C-G, U-A, R-Y, K-M, B-V, D-H; S, W to themselves), every
+ * other byte kept; written reversed into out[len].  Returns len. */
+int spoa_reverse_complement(const char* seq, uint32_t len, char* out);
+
 /* ----------------------------------------------------------- serving -- */
 /* A serving loop on one GPU (continuous batching of consensus jobs, each a
  * window of reads with its own scoring parameters; DP memory admitted per
@@ -161,6 +168,9 @@ double spoa_server_now_ms(void* server);
 /* result: status 0 ok / -1 failed; times in ms since the server started */
 int spoa_serve_result_info(void* result, int* status, double* submit_ms, double* admit_ms, double* done_ms,
                            uint32_t* rounds);
+/* the size-class lane the job ran in: 0..3 by first-read length (< 700,
+ * < 2,000, < 5,000, longer) */
+int spoa_serve_result_lane(void* result);
 const char* spoa_serve_result_error(void* result);
 int spoa_serve_result_consensus(void* result, char* out, uint32_t cap);
 int spoa_serve_result_msa(void* result, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols);

@@ -181,6 +181,10 @@ int spoa_hip_set_memory_budget(spoa_hip_engine* engine, uint64_t bytes);
 /* The current budget (by default 80% of the device memory free when the
  * engine was created). */
 uint64_t spoa_hip_memory_budget(spoa_hip_engine* engine);
+/* Device memory the engine holds for DP matrices (bytes); it stays within
+ * the budget except while one pass needs more than the segments already
+ * held (they are then released first). */
+uint64_t spoa_hip_memory_in_use(spoa_hip_engine* engine);
 
 /* DP-matrix bytes one job needs (the admission footprint used by the
  * serving loop), or 0 when the job runs no DP. */

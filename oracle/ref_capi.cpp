@@ -21,7 +21,41 @@
 
 namespace {
 thread_local std::string g_error;
+
+// The strand-ambiguous caller's reverse complement (src/main.cpp:286, 296
+// call biosoup 0.10.0 Sequence::ReverseAndComplement, which the reference
+// does not vendor).  One rule shared by the product, this oracle, the
+// reference shim and the Python harness: upper-case, then the Watson-Crick
+// / IUPAC complement (A-T, C-G, U-A, R-Y, K-M, B-V, D-H; S and W map to
+// themselves); every other character is kept as it is.  Parity with
+// biosoup is unpinned.
+char Complement(char b) {
+  switch (b) {
+    case 'A': case 'a': return 'T';
+    case 'C': case 'c': return 'G';
+    case 'G': case 'g': return 'C';
+    case 'T': case 't': case 'U': case 'u': return 'A';
+    case 'R': case 'r': return 'Y';
+    case 'Y': case 'y': return 'R';
+    case 'K': case 'k': return 'M';
+    case 'M': case 'm': return 'K';
+    case 'B': case 'b': return 'V';
+    case 'V': case 'v': return 'B';
+    case 'D': case 'd': return 'H';
+    case 'H': case 'h': return 'D';
+    case 'S': case 's': return 'S';
+    case 'W': case 'w': return 'W';
+    default: return b;
+  }
 }
+
+std::string ReverseComplement(const char* s, std::size_t len) {
+  std::string out(len, 'N');
+  for (std::size_t i = 0; i < len; ++i) out[len - 1 - i] = Complement(s[i]);
+  return out;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -124,6 +158,12 @@ int ref_graph_ranks(void* graph, std::uint32_t* out, std::uint32_t cap) {
   return static_cast<int>(r.size());
 }
 
+int ref_reverse_complement(const char* seq, std::uint32_t len, char* out) {
+  const std::string rc = ReverseComplement(seq, len);
+  if (len) std::memcpy(out, rc.data(), len);
+  return static_cast<int>(len);
+}
+
 // One window through the reference: every read aligned and added in order
 // (strand mode: the read or its reverse complement, whichever aligns with
 // the higher score, forward on ties, as src/main.cpp:285-302 does), then
@@ -143,17 +183,7 @@ int ref_run_window(int type, int m, int n, int g, int e, int q, int c, std::uint
         gr.AddAlignment(a, s, len);
         continue;
       }
-      std::string rev(s, len);
-      std::reverse(rev.begin(), rev.end());
-      for (auto& b : rev) {
-        switch (b) {
-          case 'A': b = 'T'; break;
-          case 'C': b = 'G'; break;
-          case 'G': b = 'C'; break;
-          case 'T': b = 'A'; break;
-          default: break;
-        }
-      }
+      const std::string rev = ReverseComplement(s, len);
       std::int32_t sr = 0;
       auto ar = eng->Align(rev.data(), len, gr, &sr);
       if (sf >= sr) {

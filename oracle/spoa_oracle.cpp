@@ -672,6 +672,39 @@ AlignResult Align(const Engine& en, const Graph& gr, const char* seq,
   return res;
 }
 
+// The strand-ambiguous caller's reverse complement (src/main.cpp:286, 296
+// call biosoup 0.10.0 Sequence::ReverseAndComplement, which the reference
+// does not vendor).  One rule shared by the product, this oracle, the
+// reference shim and the Python harness: upper-case, then the Watson-Crick
+// / IUPAC complement (A-T, C-G, U-A, R-Y, K-M, B-V, D-H; S and W map to
+// themselves); every other character is kept as it is.  Parity with
+// biosoup is unpinned.
+char Complement(char b) {
+  switch (b) {
+    case 'A': case 'a': return 'T';
+    case 'C': case 'c': return 'G';
+    case 'G': case 'g': return 'C';
+    case 'T': case 't': case 'U': case 'u': return 'A';
+    case 'R': case 'r': return 'Y';
+    case 'Y': case 'y': return 'R';
+    case 'K': case 'k': return 'M';
+    case 'M': case 'm': return 'K';
+    case 'B': case 'b': return 'V';
+    case 'V': case 'v': return 'B';
+    case 'D': case 'd': return 'H';
+    case 'H': case 'h': return 'D';
+    case 'S': case 's': return 'S';
+    case 'W': case 'w': return 'W';
+    default: return b;
+  }
+}
+
+std::string ReverseComplement(const char* s, std::size_t len) {
+  std::string out(len, 'N');
+  for (std::size_t i = 0; i < len; ++i) out[len - 1 - i] = Complement(s[i]);
+  return out;
+}
+
 }  // namespace oracle
 
 // ------------------------------------------------------------- C ABI ----
@@ -694,6 +727,22 @@ void* oracle_engine_new(int type, int m, int n, int g, int e, int q, int c) {
 void oracle_engine_free(void* p) { delete static_cast<Engine*>(p); }
 
 int oracle_engine_subtype(void* p) { return static_cast<Engine*>(p)->subtype; }
+
+// src/simd_alignment_engine_implementation.hpp:381-422 (Prealloc): a length
+// beyond int32 is rejected; a bound past int32 returns without sizing
+// anything; otherwise the reference sizes its buffers for that width (the
+// oracle holds none).  Returns the width it would size for (0, 16 or 32),
+// -1 on error.
+int oracle_engine_prealloc(void* p, std::uint32_t max_len, std::uint32_t alphabet) {
+  if (max_len > static_cast<std::uint32_t>(std::numeric_limits<std::int32_t>::max())) {
+    oracle::g_error = "[spoa::SimdAlignmentEngine::Prealloc] error: too large sequence!";
+    return -1;
+  }
+  const std::int64_t w = static_cast<Engine*>(p)->WorstCase(static_cast<std::int64_t>(max_len) + 8,
+                                                            static_cast<std::int64_t>(max_len) * (alphabet & 0xffu));
+  if (w < std::numeric_limits<std::int32_t>::min() + 1024) return 0;
+  return w < std::numeric_limits<std::int16_t>::min() + 1024 ? 32 : 16;
+}
 
 void* oracle_graph_new() { return new Graph(); }
 void oracle_graph_free(void* p) { delete static_cast<Graph*>(p); }
@@ -772,6 +821,12 @@ int oracle_graph_ranks(void* graph, std::uint32_t* out, std::uint32_t cap) {
 // src/main.cpp:285-302 when strand != 0), then the consensus.  Reads are
 // seqs[read_off[k], read_off[k+1]).  Returns the consensus length, -1 on
 // error, -2 if cap is too small.
+extern "C" int oracle_reverse_complement(const char* seq, std::uint32_t len, char* out) {
+  const std::string rc = oracle::ReverseComplement(seq, len);
+  if (len) std::memcpy(out, rc.data(), len);
+  return static_cast<int>(len);
+}
+
 extern "C" int oracle_run_window(int type, int m, int n, int g, int e, int q, int c, std::uint32_t n_reads,
                                  const std::uint64_t* read_off, const char* seqs, int strand, char* out,
                                  std::uint32_t cap) {
@@ -786,17 +841,7 @@ extern "C" int oracle_run_window(int type, int m, int n, int g, int e, int q, in
       const char* use = s;
       std::string rev;
       if (strand) {
-        rev.assign(s, len);
-        std::reverse(rev.begin(), rev.end());
-        for (auto& b : rev) {
-          switch (b) {
-            case 'A': b = 'T'; break;
-            case 'C': b = 'G'; break;
-            case 'G': b = 'C'; break;
-            case 'T': b = 'A'; break;
-            default: break;
-          }
-        }
+        rev = oracle::ReverseComplement(s, len);
         auto rr = oracle::Align(*en, gr, rev.data(), len);
         const std::int32_t sf = r.has_score ? r.score : 0;
         const std::int32_t sr = rr.has_score ? rr.score : 0;

@@ -172,6 +172,7 @@ def lib():
     L.spoa_server_now_ms.argtypes = [ctypes.c_void_p]
     L.spoa_serve_result_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)] + \
         [ctypes.POINTER(ctypes.c_double)] * 3 + [_u32p]
+    L.spoa_serve_result_lane.argtypes = [ctypes.c_void_p]
     L.spoa_serve_result_error.restype = ctypes.c_char_p
     L.spoa_serve_result_error.argtypes = [ctypes.c_void_p]
     L.spoa_serve_result_consensus.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32]
@@ -184,6 +185,11 @@ def lib():
     L.spoa_hip_config.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32,
                                   _i32p, _i32p]
     L.spoa_hip_set_memory_budget.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    L.spoa_hip_memory_budget.restype = ctypes.c_uint64
+    L.spoa_hip_memory_budget.argtypes = [ctypes.c_void_p]
+    L.spoa_hip_memory_in_use.restype = ctypes.c_uint64
+    L.spoa_hip_memory_in_use.argtypes = [ctypes.c_void_p]
+    L.spoa_reverse_complement.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p]
     L.spoa_hip_job_footprint.restype = ctypes.c_uint64
     L.spoa_hip_job_footprint.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]
     _lib = L
@@ -196,6 +202,16 @@ def _error():
 
 def _b(s):
     return s.encode() if isinstance(s, str) else bytes(s)
+
+
+def reverse_complement(sequence):
+    """The reverse complement that strand-ambiguous windows align (upper-case,
+    then the Watson-Crick / IUPAC complement; other bytes kept)."""
+    s = _b(sequence)
+    out = ctypes.create_string_buffer(max(1, len(s)))
+    lib().spoa_reverse_complement(s, len(s), out)
+    r = out.raw[:len(s)]
+    return r.decode() if isinstance(sequence, str) else r
 
 
 class Graph:
@@ -544,7 +560,8 @@ def _collect(h, windows, msa, strand):
 
 
 class ServeResult:
-    def __init__(self, status, error, consensus, msa, submit_ms, admit_ms, done_ms, rounds):
+    def __init__(self, status, error, consensus, msa, submit_ms, admit_ms, done_ms, rounds, lane=0):
+        self.lane = lane  # size-class lane by first-read length (< 700, < 2,000, < 5,000, longer)
         self.status = status
         self.error = error
         self.consensus = consensus
@@ -628,7 +645,8 @@ class Server:
                 break
             raw = mb.raw
             msa = [raw[i * cols.value:(i + 1) * cols.value].decode() for i in range(rows.value)]
-            return ServeResult(st.value, err, cons, msa, t[0].value, t[1].value, t[2].value, rounds.value)
+            return ServeResult(st.value, err, cons, msa, t[0].value, t[1].value, t[2].value, rounds.value,
+                               lib().spoa_serve_result_lane(r))
         finally:
             lib().spoa_serve_result_free(r)
 

@@ -144,6 +144,8 @@ Alignment HipAlignmentEngine::Align(const char* sequence, std::uint32_t sequence
     last_width_ = w < std::numeric_limits<std::int32_t>::min() + 1024
                       ? 0
                       : (w < std::numeric_limits<std::int16_t>::min() + 1024 ? 32 : 16);
+    // :723-725, before anything is sized for the call
+    if (last_width_ == 0) throw std::invalid_argument("[spoa::SimdAlignmentEngine::Align] error: possible overflow!");
   }
   const spoa_hip_graph gv = View(graph);
   const std::size_t cap = static_cast<std::size_t>(gv.num_nodes) + sequence_len;

@@ -21,10 +21,12 @@ using Clock = std::chrono::steady_clock;
 
 double Ms(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); }
 
-// IUPAC complement, upper-cased (the strand-ambiguous caller of
-// src/main.cpp:285-302 uses biosoup::Sequence::ReverseAndComplement, a
-// dependency the reference does not vendor; this restates the
-// Watson-Crick / IUPAC table).
+// The strand-ambiguous caller's complement (src/main.cpp:285-302 uses
+// biosoup::Sequence::ReverseAndComplement, a dependency the reference does
+// not vendor): upper-case, then the Watson-Crick / IUPAC complement (A-T,
+// C-G, U-A, R-Y, K-M, B-V, D-H; S and W map to themselves); every other
+// character is kept as it is.  The oracle and the test harness follow the
+// same rule.
 char Complement(char b) {
   switch (b) {
     case 'A': case 'a': return 'T';

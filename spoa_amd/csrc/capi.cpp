@@ -402,6 +402,13 @@ int spoa_batch_reversed(void* h, uint32_t w, uint8_t* out, uint32_t cap) {
   return static_cast<int>(v.size());
 }
 
+int spoa_reverse_complement(const char* seq, uint32_t len, char* out) {
+  if ((!seq || !out) && len) return -1;
+  const std::string rc = spoa_amd::ReverseComplement(seq, len);
+  if (len) std::memcpy(out, rc.data(), len);
+  return static_cast<int>(len);
+}
+
 // ----------------------------------------------------------- serving --
 
 void* spoa_server_new(const spoa_serve_opts* o) {
@@ -481,6 +488,8 @@ int spoa_serve_result_info(void* res, int* status, double* submit_ms, double* ad
   if (rounds) *rounds = r.rounds;
   return 0;
 }
+
+int spoa_serve_result_lane(void* res) { return static_cast<int>(static_cast<spoa_amd::ServeResult*>(res)->lane); }
 
 const char* spoa_serve_result_error(void* res) { return static_cast<spoa_amd::ServeResult*>(res)->error.c_str(); }
 

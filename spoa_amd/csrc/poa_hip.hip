@@ -23,6 +23,7 @@
 #include <tuple>
 #include <vector>
 
+#include "pass_split.hpp"
 #include "poa_kernels.hpp"
 #include "spoa_hip.h"
 #include "thread_pool.hpp"
@@ -932,6 +933,11 @@ int RunPassFinish(spoa_hip_engine* E) {
   return first;
 }
 
+spoa_amd::PassItem PassItemOf(const Plan& pl, const spoa_hip_job& j) {
+  return spoa_amd::PassItem{pl.mat_bytes, pl.graph_words, pl.seq_bytes,
+                            static_cast<uint64_t>(j.graph.num_nodes) + j.len};
+}
+
 int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch, bool async) {
   if (E->pending.active) {
     const int rc = RunPassFinish(E);
@@ -1000,7 +1006,7 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     const uint32_t E_in = j.graph.pred_off[V] - j.graph.pred_off[0];
     pl.graph_words = RoundUp(9ull * V + 4 + E_in, 4);
     pl.seq_bytes = RoundUp(pl.stride + 16, 16);
-    if (pl.mat_bytes > E->budget) {
+    if (!spoa_amd::PassItemFits(PassItemOf(pl, j), E->budget)) {
       fail(j, SPOA_HIP_NO_MEMORY, kMsgNoMemory);
       continue;
     }
@@ -1020,14 +1026,12 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     const uint64_t wb = static_cast<uint64_t>(jobs[b.job].graph.num_nodes) * jobs[b.job].len;
     return wa > wb;
   });
+  // passes: DP matrices within the memory budget, 32-bit offsets in range
+  std::vector<spoa_amd::PassItem> items(plans.size());
+  for (size_t x = 0; x < plans.size(); ++x) items[x] = PassItemOf(plans[x], jobs[plans[x].job]);
   size_t lo = 0;
   while (lo < plans.size()) {
-    size_t hi = lo;
-    uint64_t bytes = 0;
-    while (hi < plans.size() && bytes + plans[hi].mat_bytes <= E->budget) {
-      bytes += plans[hi].mat_bytes;
-      ++hi;
-    }
+    const size_t hi = spoa_amd::PassEnd(items.data(), lo, items.size(), E->budget);
     const bool last = hi == plans.size();
     // the pass owns a copy of the plans while it is in flight
     int rc = RunPassLaunch(E, jobs, std::vector<Plan>(plans), lo, hi);
@@ -1149,6 +1153,16 @@ int spoa_hip_prealloc(spoa_hip_engine* E, uint32_t max_len, uint32_t alphabet_si
   uint64_t largest = 0;
   for (const auto& sg : E->arena) largest = std::max(largest, sg.size);
   if (largest < bytes) {
+    // the arena stays within the budget: when the new segment would take it
+    // past, the older (smaller) segments go first -- the passes that follow
+    // grow the arena again within the budget as they need
+    if (E->arena_bytes + bytes > E->budget) {
+      if (E->pending.active) (void)RunPassFinish(E);
+      if (!HipOk(hipStreamSynchronize(E->stream), "hipStreamSynchronize")) return SPOA_HIP_DEVICE_ERROR;
+      for (auto& sg : E->arena) sg.Release();
+      E->arena.clear();
+      E->arena_bytes = 0;
+    }
     DevBuf sg;
     if (hipMalloc(&sg.p, bytes) != hipSuccess) {
       (void)hipGetLastError();
@@ -1297,6 +1311,8 @@ int spoa_hip_set_memory_budget(spoa_hip_engine* E, uint64_t bytes) {
 }
 
 uint64_t spoa_hip_memory_budget(spoa_hip_engine* E) { return E ? E->budget : 0; }
+
+uint64_t spoa_hip_memory_in_use(spoa_hip_engine* E) { return E ? E->arena_bytes : 0; }
 
 uint64_t spoa_hip_job_footprint(spoa_hip_engine* E, uint32_t V, uint32_t len, int32_t batch) {
   if (!E || V == 0 || len == 0) return 0;

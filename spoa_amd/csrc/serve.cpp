@@ -255,6 +255,7 @@ bool Server::Step(std::size_t h) {
       r.submit_ms = a.submit_ms;
       r.admit_ms = a.admit_ms;
       r.rounds = a.rounds;
+      r.lane = static_cast<std::uint32_t>(h);
       if (a.failed) {
         r.status = -1;
         r.error = a.error;
@@ -314,6 +315,7 @@ bool Server::Step(std::size_t h) {
         r.submit_ms = a->submit_ms;
         r.admit_ms = a->admit_ms;
         r.done_ms = NowMs();
+        r.lane = static_cast<std::uint32_t>(h);
         if (a->failed) {
           ++stats_.failed;
         } else {

@@ -69,6 +69,7 @@ struct ServeResult {
   double admit_ms = 0;
   double done_ms = 0;
   std::uint32_t rounds = 0;        // rounds the job took part in
+  std::uint32_t lane = 0;          // the size-class lane it ran in
 };
 
 struct ServeStats {

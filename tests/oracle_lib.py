@@ -187,7 +187,14 @@ def run_window(chk, params, reads, weights=None, strand=False, calls=None):
         chk.free_engine(eng)
 
 
-_COMP = bytes.maketrans(b"ACGTacgt", b"TGCAtgca")
+# The strand-ambiguous caller's reverse complement (src/main.cpp:286, 296
+# call biosoup's Sequence::ReverseAndComplement, which the reference does
+# not vendor): upper-case, then the Watson-Crick / IUPAC complement; every
+# other character is kept.  The same rule as the product, the oracle and
+# the reference shim; parity with biosoup is unpinned.
+_SRC = b"ACGTURYKMBVDHSW"
+_DST = b"TGCAAYRMKVBHDSW"
+_COMP = bytes.maketrans(_SRC + _SRC.lower(), _DST + _DST)
 
 
 def revcomp(s):

@@ -58,3 +58,41 @@ def test_serve_rejects_bad_parameters():
         assert r.status == 0 and r.consensus == ""
     finally:
         srv.close()
+
+
+def _job(seed, template, n_reads, err=0.10):
+    ws = spoa_amd.synth_windows(seed, 1, template=(template, template), reads=(n_reads, n_reads),
+                                err=(err, err))
+    return ws.reads(0)
+
+
+def test_serve_size_classes_at_workload_sizes():
+    """Jobs of the serving workload's sizes in every size-class lane: first
+    reads of ~400 bp, ~1.2 kb, 2-5 kb and >= 5 kb, both scorings.  Jobs up
+    to 6 kb equal the oracle run alone; a full-length job (10 kb template,
+    30 reads, int32 scores, a graph of tens of thousands of nodes) equals
+    the batched form run alone while the others share the device."""
+    sw = (spoa_amd.SW, 5, -4, -8, -6, -10, -4)
+    ov = (spoa_amd.OV, 5, -4, -8, -6)
+    jobs = [
+        (_job(1, 400, 8), sw), (_job(2, 450, 10), ov),
+        (_job(3, 1200, 10), ov), (_job(4, 1500, 8), sw),
+        (_job(5, 3000, 12), sw), (_job(6, 2500, 15), ov),
+        (_job(7, 6000, 8), ov),
+        (_job(8, 10000, 30, err=0.08), sw),
+    ]
+    lanes = [0, 0, 1, 1, 2, 2, 3, 3]
+    arrivals = [0.05 * i for i in range(len(jobs))]
+    results, wall, stats = spoa_amd.serve_trace(jobs, arrivals, threads=8)
+    assert stats["completed"] == len(jobs) and stats["failed"] == 0
+    assert [r.lane for r in results] == lanes
+    assert stats["max_active_seen"] > 1
+    for i, ((reads, params), r) in enumerate(zip(jobs, results)):
+        assert r.status == 0, (i, r.error)
+        cons, msa = _alone(reads, params)
+        assert r.consensus == cons, i
+        assert r.msa == msa, i
+        if len(reads[0]) < 8000:
+            oc, om, _ = ol.run_window(ol.oracle(), params, reads)
+            assert r.consensus == oc, i
+            assert r.msa == om, i
