@@ -97,6 +97,26 @@ int spoa_hip_prealloc(spoa_hip_engine* engine, uint32_t max_len, uint32_t alphab
 int spoa_hip_align(spoa_hip_engine* engine, const spoa_hip_graph* graph, const char* seq, uint32_t len,
                    int32_t* pairs, uint32_t cap, uint32_t* n_pairs, int32_t* score, int32_t* has_score);
 
+/* No counterpart in the reference: the variant of spoa_hip_align for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  It calls no hipMalloc, hipFree or other
+ * allocating routine and never synchronises: every device byte it needs
+ * beyond its inputs and outputs comes from a fixed scratch array of 64 MiB
+ * per GPU, held from the library's first use on that GPU until the process
+ * ends, and shared by every call on that GPU -- calls that overlap on one
+ * GPU race.  Inputs and outputs are device memory: the graph view's arrays,
+ * seq[len], pairs (2 * cap int32, 8-byte aligned; (node id, column) pairs
+ * in path order) and result[5] = {n_pairs, score, has_score, status,
+ * width_bits}, written by kernels on `stream` (a hipStream_t; NULL: the
+ * engine's stream).  Host-side errors as spoa_hip_align
+ * (SPOA_HIP_TOO_LARGE, SPOA_HIP_OVERFLOW); an alignment whose matrices
+ * and graph block need more than the scratch array fails with
+ * SPOA_HIP_NO_MEMORY before any launch (or, when its predecessor lists do
+ * not fit, with status SPOA_HIP_NO_MEMORY in result[3]); the per-alignment
+ * status of spoa_hip_align goes to result[3]. */
+int spoa_hip_align_noalloc(spoa_hip_engine* engine, const spoa_hip_graph* graph, const char* seq, uint32_t len,
+                           int32_t* pairs, uint32_t cap, int32_t* result, void* stream);
+
 /* No counterpart in the reference, which aligns one read per call: aligns
  * n_jobs independent (graph, read) pairs, one persistent launch per launch
  * configuration (and per memory-budget chunk), jobs queued by V x n.

@@ -18,6 +18,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -295,6 +296,195 @@ void FirstColumns(const spoa_hip_params& p, const spoa_hip_graph& gr, int32_t* f
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// The entry point that allocates nothing (spoa_hip_align_noalloc): every
+// device byte it uses beyond the caller's inputs and outputs -- the job
+// record, the packed graph block, the padded read, the DP matrices, the
+// queue counter and the kernel's result record -- lives in this one
+// file-scope array.  Each GPU holds its own copy from the first use of the
+// library on it until the process ends; every call on one GPU shares it, so
+// calls that overlap on one GPU race.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in): the device address
+// of each GPU's copy comes from hipGetSymbolAddress at engine creation.
+constexpr uint64_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[kMaxDevices] = {};
+
+// The scratch array's device address on `device`, which must be current.
+int ScratchInit(int device) {
+  if (device < 0 || device >= kMaxDevices) return Fail(SPOA_HIP_DEVICE_ERROR, "[spoa::hip] device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return SPOA_HIP_OK;
+  void* ptr = nullptr;
+  if (!HipOk(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) {
+    return SPOA_HIP_DEVICE_ERROR;
+  }
+  g_scratch[device] = static_cast<uint8_t*>(ptr);
+  return SPOA_HIP_OK;
+}
+
+}  // namespace
+
+namespace spoa_hip_dev {
+
+// Arguments of the packing kernel of the no-allocation entry point.
+struct NoAllocPackArgs {
+  // the caller's rank view and read (device memory)
+  const uint8_t* row_char;
+  const int32_t* node_id;
+  const uint32_t* pred_off;
+  const uint32_t* pred_rank;
+  const uint8_t* is_sink;
+  const char* seq;
+  uint32_t V, n, W, stride, cap;
+  int32_t type, subtype, g, e, q, c;
+  // inside the scratch array
+  uint32_t* counter;
+  DevJob* job;
+  int32_t* res;       // the kernel's result record: n_pairs, score, has_score, status
+  uint8_t* seq_out;   // stride + 16 bytes
+  uint64_t mat_addr;  // DP matrices
+  uint32_t* graph;    // graph block (layout of MakeView)
+  uint64_t graph_cap; // 32-bit words available for the graph block
+};
+
+// Builds the job record, the graph block (row records, node ids, predecessor
+// offsets, first columns, predecessor rows) and the zero-padded read in the
+// scratch array, and arms the queue counter: 0 when the job fits, 1 (the
+// alignment kernel then takes no job) with status "insufficient memory"
+// when its predecessor lists do not fit.  First columns as FirstColumns on
+// the host (reference simd...:568-577, 590-599, 607-694): one thread walks
+// the ranks in order.  One workgroup.
+__global__ void NoAllocPack(NoAllocPackArgs a) {
+  const uint32_t V = a.V;
+  const uint32_t base = a.pred_off[0];
+  const uint64_t E = static_cast<uint64_t>(a.pred_off[V]) - base;
+  const bool fits = 9ull * V + 4 + E <= a.graph_cap;
+  if (!fits) {
+    if (threadIdx.x == 0) {
+      *a.counter = 1u;
+      a.res[0] = 0;
+      a.res[1] = 0;
+      a.res[2] = 0;
+      a.res[3] = SPOA_HIP_NO_MEMORY;
+    }
+    return;
+  }
+  uint32_t* rows = a.graph;
+  int32_t* node_id = reinterpret_cast<int32_t*>(a.graph + 4ull * V);
+  uint32_t* poff = a.graph + 5ull * V;
+  int32_t* fcH = reinterpret_cast<int32_t*>(a.graph + 6ull * V + 1);
+  int32_t* fcF = fcH + (V + 1);
+  int32_t* fcO = fcF + (V + 1);
+  uint32_t* preds = a.graph + 9ull * V + 4;
+  if (threadIdx.x == 0) {
+    const int32_t kNinf = -2147483647 - 1 + 1024;
+    if (a.type != SPOA_HIP_NW) {
+      for (uint32_t i = 0; i <= V; ++i) fcH[i] = fcF[i] = fcO[i] = 0;
+    } else {
+      fcF[0] = 0;
+      fcO[0] = 0;
+      for (uint32_t i = 1; i <= V; ++i) {
+        const uint32_t b = a.pred_off[i - 1], e = a.pred_off[i];
+        int32_t pf = b == e ? a.g - a.e : kNinf;
+        int32_t po = b == e ? a.q - a.c : kNinf;
+        for (uint32_t k = b; k < e; ++k) {
+          const uint32_t row = a.pred_rank[k] + 1;
+          pf = max(pf, fcF[row]);
+          po = max(po, fcO[row]);
+        }
+        fcF[i] = pf + a.e;
+        fcO[i] = po + a.c;
+      }
+      if (a.subtype == 2) {
+        for (uint32_t i = 0; i <= V; ++i) fcH[i] = max(fcF[i], fcO[i]);
+      } else if (a.subtype == 1) {
+        for (uint32_t i = 0; i <= V; ++i) fcH[i] = fcF[i];
+      } else {
+        fcH[0] = 0;
+        for (uint32_t i = 1; i <= V; ++i) {
+          const uint32_t b = a.pred_off[i - 1], e = a.pred_off[i];
+          int32_t ph = b == e ? 0 : kNinf;
+          for (uint32_t k = b; k < e; ++k) ph = max(ph, fcH[a.pred_rank[k] + 1]);
+          fcH[i] = ph + a.g;
+        }
+      }
+    }
+    DevJob j{};
+    j.mat_addr = a.mat_addr;
+    j.graph_off = 0;
+    j.seq_off = 0;
+    j.V = V;
+    j.n = a.n;
+    j.W = a.W;
+    j.stride = a.stride;
+    j.pair_off = 0;
+    j.cap = a.cap;
+    j.nslots = 0;  // the query profile is formed per row
+    j.slot_chars = 0;
+    *a.job = j;
+    *a.counter = 0u;
+    a.res[0] = 0;
+    a.res[1] = 0;
+    a.res[2] = 0;
+    a.res[3] = 0;
+  }
+  __syncthreads();
+  for (uint32_t r = threadIdx.x; r < V; r += blockDim.x) {
+    const uint32_t pb = a.pred_off[r] - base, pe = a.pred_off[r + 1] - base;
+    const uint32_t npred = pe > pb ? pe - pb : 1u;
+    rows[4ull * r + 0] = static_cast<uint32_t>(a.row_char[r]) | (a.is_sink[r] ? 0x100u : 0u) | (7u << 9) |
+                         (npred << 12);
+    rows[4ull * r + 1] = pe > pb ? a.pred_rank[a.pred_off[r]] + 1 : 0u;
+    rows[4ull * r + 2] = pb;
+    rows[4ull * r + 3] = static_cast<uint32_t>(fcH[r + 1]);
+    node_id[r] = a.node_id[r];
+    poff[r] = pb;
+  }
+  if (threadIdx.x == 0) poff[V] = static_cast<uint32_t>(E);
+  for (uint64_t k = threadIdx.x; k < E; k += blockDim.x) preds[k] = a.pred_rank[base + k] + 1;
+  for (uint32_t c = threadIdx.x; c < a.stride + 16; c += blockDim.x) {
+    a.seq_out[c] = c < a.n ? static_cast<uint8_t>(a.seq[c]) : 0u;
+  }
+}
+
+// Turns the alignment kernel's result into the caller's record
+// {n_pairs, score, has_score, status, width_bits} and reverses the pairs
+// (stored in traceback order) into path order, in place.
+__global__ void NoAllocFinish(const int32_t* res, int32_t* pairs, uint32_t cap, int32_t width_bits,
+                              int32_t* result) {
+  const int32_t status = res[3];
+  const uint32_t np = static_cast<uint32_t>(res[0]);
+  const bool ok = status == 0 && np <= cap;
+  if (ok) {
+    for (uint32_t k = threadIdx.x; k < np / 2; k += blockDim.x) {
+      const int2 x = reinterpret_cast<int2*>(pairs)[k];
+      const int2 y = reinterpret_cast<int2*>(pairs)[np - 1 - k];
+      reinterpret_cast<int2*>(pairs)[k] = y;
+      reinterpret_cast<int2*>(pairs)[np - 1 - k] = x;
+    }
+  }
+  if (threadIdx.x == 0) {
+    result[0] = ok ? static_cast<int32_t>(np) : 0;
+    result[1] = res[1];
+    result[2] = ok ? res[2] : 0;
+    result[3] = status != 0 ? status : (np <= cap ? 0 : SPOA_HIP_CAPACITY);
+    result[4] = width_bits;
+  }
+}
+
+// The caller's result record when no DP runs (empty graph or read).
+__global__ void NoAllocEmpty(int32_t* result) {
+  if (threadIdx.x < 5) result[threadIdx.x] = 0;
+}
+
+}  // namespace spoa_hip_dev
 
 struct Plan {
   uint32_t job;
@@ -1068,6 +1258,7 @@ int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_e
   if (!HipOk(hipGetDeviceCount(&count), "hipGetDeviceCount")) return SPOA_HIP_DEVICE_ERROR;
   if (device < 0 || device >= count) return Fail(SPOA_HIP_DEVICE_ERROR, "[spoa::hip] no such HIP device");
   if (!HipOk(hipSetDevice(device), "hipSetDevice")) return SPOA_HIP_DEVICE_ERROR;
+  if (ScratchInit(device) != SPOA_HIP_OK) return SPOA_HIP_DEVICE_ERROR;
   auto* E = new spoa_hip_engine();
   E->device = device;
   E->prm = *params;
@@ -1190,6 +1381,86 @@ int spoa_hip_align(spoa_hip_engine* E, const spoa_hip_graph* graph, const char* 
   if (has_score) *has_score = j.has_score;
   if (score && j.has_score) *score = j.score;
   return rc;
+}
+
+int spoa_hip_align_noalloc(spoa_hip_engine* E, const spoa_hip_graph* graph, const char* seq, uint32_t len,
+                           int32_t* pairs, uint32_t cap, int32_t* result, void* stream) {
+  if (!E || !graph || !result || (!seq && len) || (!pairs && cap)) {
+    return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null argument");
+  }
+  if (!HipOk(hipSetDevice(E->device), "hipSetDevice")) return SPOA_HIP_DEVICE_ERROR;
+  uint8_t* const base = E->device < kMaxDevices ? g_scratch[E->device] : nullptr;
+  if (!base) return Fail(SPOA_HIP_DEVICE_ERROR, "[spoa::hip] no scratch address for the device");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : E->stream;
+  // reference simd...:708-715
+  if (len > static_cast<uint32_t>(std::numeric_limits<int32_t>::max())) return Fail(SPOA_HIP_TOO_LARGE, kMsgTooLarge);
+  const uint32_t V = graph->num_nodes;
+  if (V == 0 || len == 0) {
+    spoa_hip_dev::NoAllocEmpty<<<1, 64, 0, st>>>(result);
+    return HipOk(hipGetLastError(), "kernel launch") ? SPOA_HIP_OK : SPOA_HIP_DEVICE_ERROR;
+  }
+  const Shape sh = JobShape(E->prm, len, V);
+  if (sh.width == 0) return Fail(SPOA_HIP_OVERFLOW, kMsgOverflow);
+  // the per-call launch configuration of spoa_hip_align
+  const Cfg cfg = ChooseCfg(E, len, sh.width, false);
+  const uint32_t W = (len + sh.lanes - 1) / sh.lanes * sh.lanes;
+  const uint64_t stride = RoundUp(W, static_cast<uint64_t>(cfg.k));
+  // scratch layout: counter | job record | result record | read | matrices | graph block
+  const uint64_t off_seq = 768;
+  const uint64_t off_mat = RoundUp(off_seq + stride + 16, kAlign);
+  const uint64_t mat_bytes = JobBytes(E->prm, V, static_cast<uint32_t>(stride), sh.bits, cfg);
+  const uint64_t off_graph = RoundUp(off_mat + mat_bytes, kAlign);
+  if (off_graph + (9ull * V + 4) * 4 > kScratchBytes) {
+    return Fail(SPOA_HIP_NO_MEMORY,
+                "[spoa::hip] the alignment needs " + std::to_string(off_graph + (9ull * V + 4) * 4) +
+                    " bytes of device scratch, more than the " + std::to_string(kScratchBytes >> 20) +
+                    " MiB the no-allocation entry point has");
+  }
+  KernelFn fn = PickKernel(E->prm.subtype, sh.bits, sh.lanes, cfg.k, cfg.waves);
+  if (!fn) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] unsupported launch configuration");
+  spoa_hip_dev::NoAllocPackArgs pa{};
+  pa.row_char = graph->row_char;
+  pa.node_id = graph->node_id;
+  pa.pred_off = graph->pred_off;
+  pa.pred_rank = graph->pred_rank;
+  pa.is_sink = graph->is_sink;
+  pa.seq = seq;
+  pa.V = V;
+  pa.n = len;
+  pa.W = W;
+  pa.stride = static_cast<uint32_t>(stride);
+  pa.cap = cap;
+  pa.type = E->prm.type;
+  pa.subtype = E->prm.subtype;
+  pa.g = E->prm.g;
+  pa.e = E->prm.e;
+  pa.q = E->prm.q;
+  pa.c = E->prm.c;
+  pa.counter = reinterpret_cast<uint32_t*>(base);
+  pa.job = reinterpret_cast<DevJob*>(base + 256);
+  pa.res = reinterpret_cast<int32_t*>(base + 512);
+  pa.seq_out = base + off_seq;
+  pa.mat_addr = reinterpret_cast<uint64_t>(base + off_mat);
+  pa.graph = reinterpret_cast<uint32_t*>(base + off_graph);
+  pa.graph_cap = (kScratchBytes - off_graph) / 4;
+  spoa_hip_dev::NoAllocPack<<<1, 256, 0, st>>>(pa);
+  if (!HipOk(hipGetLastError(), "kernel launch")) return SPOA_HIP_DEVICE_ERROR;
+  KArgs args{};
+  args.prm = MakeParams(E->prm, sh.bits, sh.lanes, cfg.k);
+  args.jobs = pa.job;
+  args.n_jobs = 1;
+  args.queue = pa.counter;
+  args.graphs = pa.graph;
+  args.seqs = pa.seq_out;
+  args.pairs = pairs;
+  args.results = pa.res;
+  const int threads = 64 * cfg.waves;
+  args.lds_bytes = 8u * static_cast<uint32_t>(cfg.k) * static_cast<uint32_t>(threads);
+  args.prio_jobs = 0;
+  fn<<<1, threads, args.lds_bytes, st>>>(args);
+  if (!HipOk(hipGetLastError(), "kernel launch")) return SPOA_HIP_DEVICE_ERROR;
+  spoa_hip_dev::NoAllocFinish<<<1, 256, 0, st>>>(pa.res, pairs, cap, sh.width, result);
+  return HipOk(hipGetLastError(), "kernel launch") ? SPOA_HIP_OK : SPOA_HIP_DEVICE_ERROR;
 }
 
 int spoa_hip_align_batch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs) {
