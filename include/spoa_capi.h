@@ -77,6 +77,7 @@ typedef struct spoa_batch_opts {
   int32_t device;
   int32_t overlap; /* batched form: the two halves' launches may run on the device at once */
   int32_t graph;   /* batched form: every pass launched as a captured HIP graph */
+  int32_t resident; /* batched form: window graphs on the device (merges, consensus and MSA there) */
 } spoa_batch_opts;
 
 typedef struct spoa_batch_stats {

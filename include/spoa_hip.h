@@ -159,6 +159,8 @@ typedef struct spoa_hip_stats {
   double kernel_ms32;  /* kernel_ms of the int32-score launches */
   uint64_t dp_bytes16, dp_bytes32;  /* dp_bytes of the int16 / int32 jobs */
   uint32_t launches16, launches32;  /* launches of int16 / int32 jobs */
+  double merge_ms;   /* device-resident windows: device time of the graph merges */
+  uint32_t merges;   /* device-resident windows: windows merged */
 } spoa_hip_stats;
 int spoa_hip_last_stats(spoa_hip_engine* engine, spoa_hip_stats* out);
 
@@ -209,6 +211,51 @@ uint64_t spoa_hip_memory_in_use(spoa_hip_engine* engine);
 /* DP-matrix bytes one job needs (the admission footprint used by the
  * serving loop), or 0 when the job runs no DP. */
 uint64_t spoa_hip_job_footprint(spoa_hip_engine* engine, uint32_t num_nodes, uint32_t len, int32_t batch);
+
+/* ------------------------------------------------------------------
+ * Device-resident windows.  No counterpart in the reference, whose graphs
+ * live on the host: every window's POA graph stays in device memory for the
+ * run.  Round r aligns read r of every window that has one (the engine's
+ * batched alignment kernels; both strands in strand mode, the better kept
+ * as src/main.cpp:285-302 does), then merges each alignment into its
+ * window's graph on the device -- Graph::AddAlignment, TopologicalSort and
+ * the rank-ordered view of the next round (reference src/graph.cpp:76-303)
+ * -- one wavefront a window.  The finish generates every window's consensus
+ * and MSA on the device (src/graph.cpp:321-516).  Reads as
+ * spoa_batch_run's: window w holds reads [win_read_off[w],
+ * win_read_off[w + 1]), read k is seqs[read_off[k], read_off[k + 1]), quals
+ * likewise (Phred+33 weights) or NULL.  Results equal the reference's
+ * window by window.  Everything runs on the engine's stream. */
+typedef struct spoa_hip_windows spoa_hip_windows;
+int spoa_hip_windows_create(spoa_hip_engine* engine, uint32_t n_windows, const uint32_t* win_read_off,
+                            const uint64_t* read_off, const char* seqs, const char* quals, int32_t strand,
+                            spoa_hip_windows** out);
+int spoa_hip_windows_destroy(spoa_hip_windows* windows);
+/* the number of rounds: the most reads of one window */
+uint32_t spoa_hip_windows_rounds(spoa_hip_windows* windows);
+/* launches round r (rounds in order, each after the previous one's wait) */
+int spoa_hip_windows_submit(spoa_hip_windows* windows, uint32_t round);
+/* 1 when the submitted round has finished on the device (wait will not
+ * block), 0 while it runs */
+int spoa_hip_windows_ready(spoa_hip_windows* windows);
+/* waits for the submitted round; a failed window's message names it */
+int spoa_hip_windows_wait(spoa_hip_windows* windows);
+/* device times and counts of the rounds so far */
+int spoa_hip_windows_stats(spoa_hip_windows* windows, spoa_hip_stats* out);
+/* consensus (and with msa != 0 the MSA) of every window, held by the
+ * context until it is destroyed */
+int spoa_hip_windows_finish(spoa_hip_windows* windows, int32_t msa);
+int spoa_hip_windows_consensus(spoa_hip_windows* windows, uint32_t window, const char** out, uint32_t* len);
+/* n_rows rows of width bytes back to back (sequences in insertion order) */
+int spoa_hip_windows_msa(spoa_hip_windows* windows, uint32_t window, const char** rows, uint32_t* n_rows,
+                         uint32_t* width);
+/* strand mode: 1 per read where the reverse complement was added; returns
+ * the read count (negative status on error) */
+int spoa_hip_windows_reversed(spoa_hip_windows* windows, uint32_t window, uint8_t* out, uint32_t cap);
+/* Graph::rank_to_node() ids of a window after the last finished round;
+ * returns V (negative status on error) */
+int spoa_hip_windows_ranks(spoa_hip_windows* windows, uint32_t window, uint32_t* out, uint32_t cap);
+const char* spoa_hip_windows_last_error(spoa_hip_windows* windows);
 
 const char* spoa_hip_last_error(void);
 

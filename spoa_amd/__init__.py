@@ -46,13 +46,14 @@ class HipStats(ctypes.Structure):
                 ("cells", ctypes.c_uint64), ("dp_bytes", ctypes.c_uint64),
                 ("kernel_ms16", ctypes.c_double), ("kernel_ms32", ctypes.c_double),
                 ("dp_bytes16", ctypes.c_uint64), ("dp_bytes32", ctypes.c_uint64),
-                ("launches16", ctypes.c_uint32), ("launches32", ctypes.c_uint32)]
+                ("launches16", ctypes.c_uint32), ("launches32", ctypes.c_uint32),
+                ("merge_ms", ctypes.c_double), ("merges", ctypes.c_uint32)]
 
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int32) for k in
                 ("type", "m", "n", "g", "e", "q", "c", "strand", "msa", "threads", "device", "overlap",
-                 "graph")]
+                 "graph", "resident")]
 
 
 class BatchStats(ctypes.Structure):
@@ -478,13 +479,14 @@ class WindowResults:
             self._h = None
 
 
-def _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap=False, graph=False):
+def _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap=False, graph=False, resident=False):
     if e is None:
         e = g
     if q is None:
         q, c = g, e
     return BatchOpts(type=type, m=m, n=n, g=g, e=e, q=q, c=c, strand=int(strand), msa=int(msa),
-                     threads=threads, device=device, overlap=int(overlap), graph=int(graph))
+                     threads=threads, device=device, overlap=int(overlap), graph=int(graph),
+                     resident=int(resident))
 
 
 class BatchContext:
@@ -492,8 +494,8 @@ class BatchContext:
     between runs) for one set of scoring options."""
 
     def __init__(self, type, m, n, g, e=None, q=None, c=None, strand=False, msa=True, threads=0, device=0,
-                 overlap=False, graph=False):
-        self.opts = _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap, graph)
+                 overlap=False, graph=False, resident=False):
+        self.opts = _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap, graph, resident)
         self._h = lib().spoa_batch_context_new(ctypes.byref(self.opts))
         if not self._h:
             raise ValueError(_error())
@@ -541,11 +543,12 @@ def run_windows(windows, type, m, n, g, e=None, q=None, c=None, mode="batched", 
                 msa=True, threads=0, device=0):
     """Aligns every read of every window in order into the window's graph,
     then builds its consensus (and MSA).  mode 'batched': one launch per
-    round over all windows; 'per_call': the drop-in Align, one read at a
-    time."""
+    round over all windows, graphs merged on the host; 'resident': the same
+    launches with every window's graph kept, merged, sorted and summarised
+    on the device; 'per_call': the drop-in Align, one read at a time."""
     if not isinstance(windows, Windows):
         windows = Windows.from_lists(windows)
-    o = _opts(type, m, n, g, e, q, c, strand, msa, threads, device)
+    o = _opts(type, m, n, g, e, q, c, strand, msa, threads, device, resident=mode == "resident")
     md = 1 if mode == "per_call" else 0
     h = lib().spoa_batch_run(md, ctypes.byref(o), windows.n_windows,
                              windows.win_read_off.ctypes.data, windows.read_off.ctypes.data,

@@ -127,7 +127,131 @@ struct Half {
 
 }  // namespace
 
+std::vector<WindowResult> RunResident(const WindowSet& ws, BatchContext& ctx, BatchStats* stats) {
+  const auto t0 = Clock::now();
+  BatchStats st;
+  const BatchOptions& opt = ctx.options();
+  const std::uint32_t W = ws.n_windows;
+  std::vector<WindowResult> out(W);
+  // two contiguous halves of about equal bases
+  std::uint32_t split = 0;
+  {
+    const std::uint64_t all = ws.read_off[ws.win_read_off[W]] - ws.read_off[ws.win_read_off[0]];
+    while (split < W && 2 * (ws.read_off[ws.win_read_off[split]] - ws.read_off[ws.win_read_off[0]]) < all) ++split;
+    if (W > 1) split = std::max<std::uint32_t>(1, std::min(split, W - 1));
+  }
+  struct Part {
+    spoa_hip_engine* dev = nullptr;
+    spoa_hip_windows* win = nullptr;
+    std::uint32_t first = 0, count = 0, rounds = 0;
+    bool pending = false;
+  } part[2];
+  auto fail = [&](spoa_hip_windows* x, int rc) {
+    std::string msg = x ? spoa_hip_windows_last_error(x) : "";
+    if (msg.empty()) msg = spoa_hip_last_error();
+    for (auto& p : part) {
+      if (p.win) spoa_hip_windows_destroy(p.win);
+      p.win = nullptr;
+    }
+    throw std::invalid_argument(msg.empty() ? "[spoa::hip] device-resident windows failed (" + std::to_string(rc) + ")"
+                                            : msg);
+  };
+  for (int h = 0; h < 2; ++h) {
+    auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(ctx.engine(h));
+    if (!hip) throw std::invalid_argument("[spoa::hip] the alignment engine is not the HIP engine");
+    Part& p = part[h];
+    p.dev = hip->device();
+    p.first = h == 0 ? 0 : split;
+    p.count = h == 0 ? split : W - split;
+    if (p.count == 0) continue;
+    const int rc = spoa_hip_windows_create(p.dev, p.count, ws.win_read_off + p.first, ws.read_off, ws.seqs, ws.quals,
+                                           opt.strand ? 1 : 0, &p.win);
+    if (rc != SPOA_HIP_OK) fail(nullptr, rc);
+    p.rounds = spoa_hip_windows_rounds(p.win);
+    if (opt.max_rounds > 0) p.rounds = std::min(p.rounds, opt.max_rounds);
+  }
+  const std::uint32_t rounds = std::max(part[0].rounds, part[1].rounds);
+  for (std::uint32_t r = 0; r < rounds; ++r) {
+    for (auto& p : part) {
+      if (!p.win) continue;
+      if (p.pending) {
+        const auto ta = Clock::now();
+        const int rc = spoa_hip_windows_wait(p.win);
+        st.wait_ms += Ms(ta, Clock::now());
+        p.pending = false;
+        if (rc != SPOA_HIP_OK) fail(p.win, rc);
+      }
+      if (r < p.rounds) {
+        const auto ta = Clock::now();
+        const int rc = spoa_hip_windows_submit(p.win, r);
+        st.align_ms += Ms(ta, Clock::now());
+        if (rc != SPOA_HIP_OK) fail(p.win, rc);
+        p.pending = true;
+      }
+    }
+    ++st.rounds;
+  }
+  for (auto& p : part) {
+    if (p.win && p.pending) {
+      const auto ta = Clock::now();
+      const int rc = spoa_hip_windows_wait(p.win);
+      st.wait_ms += Ms(ta, Clock::now());
+      if (rc != SPOA_HIP_OK) fail(p.win, rc);
+    }
+  }
+  const auto tf = Clock::now();
+  const bool finish = opt.max_rounds == 0;
+  for (auto& p : part) {
+    if (!p.win) continue;
+    if (finish) {
+      const int rc = spoa_hip_windows_finish(p.win, opt.msa ? 1 : 0);
+      if (rc != SPOA_HIP_OK) fail(p.win, rc);
+      for (std::uint32_t i = 0; i < p.count; ++i) {
+        WindowResult& o = out[p.first + i];
+        const char* c = nullptr;
+        std::uint32_t len = 0;
+        spoa_hip_windows_consensus(p.win, i, &c, &len);
+        o.consensus.assign(c, len);
+        if (opt.msa) {
+          const char* rows = nullptr;
+          std::uint32_t nr = 0, width = 0;
+          spoa_hip_windows_msa(p.win, i, &rows, &nr, &width);
+          o.msa.resize(nr);
+          for (std::uint32_t k = 0; k < nr; ++k) o.msa[k].assign(rows + static_cast<std::size_t>(k) * width, width);
+        }
+        if (opt.strand) {
+          const std::uint32_t n = ws.win_read_off[p.first + i + 1] - ws.win_read_off[p.first + i];
+          o.reversed.resize(n);
+          if (spoa_hip_windows_reversed(p.win, i, o.reversed.data(), n) < 0) fail(p.win, -1);
+        }
+      }
+    }
+    spoa_hip_stats hs{};
+    spoa_hip_windows_stats(p.win, &hs);
+    st.kernel_ms += hs.kernel_ms;
+    st.kernel_ms16 += hs.kernel_ms16;
+    st.kernel_ms32 += hs.kernel_ms32;
+    st.dp_bytes16 += hs.dp_bytes16;
+    st.dp_bytes32 += hs.dp_bytes32;
+    st.launches16 += hs.launches16;
+    st.launches32 += hs.launches32;
+    st.launches += hs.launches;
+    st.cells += hs.cells;
+    st.dp_bytes += hs.dp_bytes;
+    st.align_calls += hs.jobs;
+    st.pack_ms += hs.host_ms;
+    st.graph_ms += hs.merge_ms;  // device time of the merges
+    spoa_hip_windows_destroy(p.win);
+    p.win = nullptr;
+  }
+  st.finish_ms = Ms(tf, Clock::now());
+  st.wall_ms = Ms(t0, Clock::now());
+  if (stats) *stats = st;
+  return out;
+}
+
 std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, BatchStats* stats) {
+  if (ctx.options().resident) return RunResident(ws, ctx, stats);
   const auto t0 = Clock::now();
   BatchStats st;
   const BatchOptions& opt = ctx.options();

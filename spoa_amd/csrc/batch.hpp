@@ -42,6 +42,7 @@ struct BatchOptions {
   bool overlap = false;    // the two halves' launches may share the device (else they take turns)
   std::uint32_t max_rounds = 0;  // stop after this many rounds (0: all; the autotuner's short runs)
   bool graph = false;      // launch every pass as a captured HIP graph
+  bool resident = false;   // window graphs on the device: merges, consensus and MSA there too
   unsigned threads = 0;    // host threads (0: all hardware threads)
   int device = 0;
 };
@@ -95,9 +96,16 @@ class BatchContext {
 };
 
 // Batched form.  Throws std::invalid_argument with the engine's message on
-// failure.
+// failure.  With options().resident the window graphs stay on the device
+// (RunResident).
 std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, BatchStats* stats);
 std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats);
+
+// Batched form with device-resident window graphs (spoa_hip_windows_*): the
+// windows split in two contiguous halves of about equal bases, an engine
+// and a stream each, rounds alternating between them so that one half's
+// merges run beside the other half's alignments.
+std::vector<WindowResult> RunResident(const WindowSet& ws, BatchContext& ctx, BatchStats* stats);
 
 // Autotuner of the batched launches: for each score width, tries launch
 // configurations (columns per lane, waves per workgroup) on the windows

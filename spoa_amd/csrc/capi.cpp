@@ -58,6 +58,7 @@ spoa_amd::BatchOptions Options(const spoa_batch_opts* o) {
   b.device = o->device;
   b.overlap = o->overlap != 0;
   b.graph = o->graph != 0;
+  b.resident = o->resident != 0;
   return b;
 }
 

@@ -24,152 +24,15 @@
 #include <tuple>
 #include <vector>
 
+#include "engine.hpp"
 #include "pass_split.hpp"
 #include "poa_kernels.hpp"
 #include "spoa_hip.h"
 #include "thread_pool.hpp"
 
-using spoa_hip_dev::DevJob;
-using spoa_hip_dev::KArgs;
-using spoa_hip_dev::KParams;
+using namespace spoa_hip_host;
 
-namespace {
-
-thread_local std::string g_last_error;
-
-int Fail(int code, const std::string& msg) {
-  g_last_error = msg;
-  return code;
-}
-
-bool HipOk(hipError_t rc, const char* what) {
-  if (rc == hipSuccess) return true;
-  g_last_error = std::string("[spoa::hip] ") + what + ": " + hipGetErrorString(rc);
-  return false;
-}
-
-const char kMsgTooLarge[] = "[spoa::SimdAlignmentEngine::Align] error: too large sequence!";
-const char kMsgOverflow[] = "[spoa::SimdAlignmentEngine::Align] error: possible overflow!";
-const char kMsgNoMemory[] = "[spoa::SimdAlignmentEngine::Align] error: insufficient memory!";
-const char kMsgPreTooLarge[] = "[spoa::SimdAlignmentEngine::Prealloc] error: too large sequence!";
-const char kMsgPreNoMemory[] = "[spoa::SimdAlignmentEngine::Prealloc] error: insufficient memory!";
-
-constexpr uint64_t kAlign = 256;
-uint64_t RoundUp(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
-
-// reference src/alignment_engine.cpp:101-110
-int64_t WorstCase(const spoa_hip_params& p, int64_t i, int64_t j) {
-  auto gap = [&](int64_t len) -> int64_t {
-    return len == 0 ? 0 : std::min<int64_t>(p.g + (len - 1) * p.e, p.q + (len - 1) * p.c);
-  };
-  const int64_t d = i > j ? i - j : j - i;
-  return std::min<int64_t>(-1 * (p.m * std::min(i, j) + gap(d)), gap(i) + gap(j));
-}
-
-// Score width for one call (reference simd...:719-774): 0 means "possible
-// overflow!".
-int WidthBits(const spoa_hip_params& p, uint32_t len, uint32_t V) {
-  const int64_t w = WorstCase(p, static_cast<int64_t>(len) + 8, V);
-  if (w < std::numeric_limits<int32_t>::min() + 1024) return 0;
-  if (w < std::numeric_limits<int16_t>::min() + 1024) return 32;
-  return 16;
-}
-
-int Lanes(int width_bits) { return width_bits == 16 ? 16 : 8; }
-
-// How one job runs: `width` is the reference's score width (what the caller
-// is told), `bits` the width of the stored rows, `lanes` the reference lane
-// count whose row width and convex segments the DP reproduces.  Local
-// alignments that the reference runs at int32 keep int16 rows when every
-// score fits them: H stays in [0, m * n] and every gap score is at least
-// its penalty above a non-negative H, so only m * (n + 8) needs headroom
-// (kept at 30000 below int16's maximum).
-struct Shape {
-  int width = 0, bits = 0, lanes = 0;
-};
-
-Shape JobShape(const spoa_hip_params& p, uint32_t len, uint32_t V) {
-  Shape sh;
-  sh.width = WidthBits(p, len, V);
-  if (sh.width == 0) return sh;
-  sh.lanes = Lanes(sh.width);
-  sh.bits = sh.width;
-  if (sh.width == 32 && p.type == SPOA_HIP_SW &&
-      static_cast<int64_t>(std::max<int32_t>(p.m, 0)) * (static_cast<int64_t>(len) + 8) <= 30000) {
-    sh.bits = 16;
-  }
-  return sh;
-}
-int Matrices(int subtype) { return subtype == 0 ? 1 : (subtype == 1 ? 3 : 5); }
-int ReadMatrices(int subtype) { return subtype == 0 ? 1 : (subtype == 1 ? 2 : 3); }
-
-struct Cfg {
-  int k = 8;
-  int waves = 1;
-  bool operator<(const Cfg& o) const { return std::tie(k, waves) < std::tie(o.k, o.waves); }
-};
-
-// Device buffer, grow-only.
-struct DevBuf {
-  void* p = nullptr;
-  uint64_t size = 0;
-  bool Reserve(uint64_t need, const char* what) {
-    if (need <= size) return true;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    size = 0;
-    const uint64_t want = std::max<uint64_t>(need, std::min<uint64_t>(2 * need, std::max<uint64_t>(size * 2, need + need / 4)));
-    if (hipMalloc(&p, want) == hipSuccess) {
-      size = want;
-      return true;
-    }
-    (void)hipGetLastError();
-    if (hipMalloc(&p, need) == hipSuccess) {
-      size = need;
-      return true;
-    }
-    (void)hipGetLastError();
-    p = nullptr;
-    g_last_error = what;
-    return false;
-  }
-  void Release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    size = 0;
-  }
-};
-
-// Pinned host buffer, grow-only.
-struct PinBuf {
-  void* p = nullptr;
-  uint64_t size = 0;
-  bool Reserve(uint64_t need) {
-    if (need <= size) return true;
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    size = 0;
-    const uint64_t want = std::max<uint64_t>(need, 2 * need);
-    if (!HipOk(hipHostMalloc(&p, want, hipHostMallocDefault), "hipHostMalloc")) return false;
-    size = want;
-    return true;
-  }
-  void Release() {
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    size = 0;
-  }
-  template <typename T>
-  T* At(uint64_t off) {
-    return reinterpret_cast<T*>(static_cast<uint8_t*>(p) + off);
-  }
-};
-
-using KernelFn = void (*)(KArgs);
-
-// Multi-wave workgroups have at most 8 waves (two a SIMD), which leaves
-// each lane 256 registers.
-constexpr int kMaxGroupWaves = 8;
+namespace spoa_hip_host {
 
 template <int SUB, typename ST, int K, int L>
 KernelFn PickMaxT(int waves) {
@@ -208,45 +71,140 @@ KernelFn PickKernel(int subtype, int bits, int lanes, int k, int waves) {
   }
 }
 
-// max-plus 2x2 product
-void MpMul(const int32_t* a, const int32_t* b, int32_t* out) {
-  int32_t r[4];
-  r[0] = std::max(a[0] + b[0], a[1] + b[2]);
-  r[1] = std::max(a[0] + b[1], a[1] + b[3]);
-  r[2] = std::max(a[2] + b[0], a[3] + b[2]);
-  r[3] = std::max(a[2] + b[1], a[3] + b[3]);
-  std::memcpy(out, r, sizeof(r));
+
+bool LaunchResetCounters(uint32_t* counters, uint32_t n, hipStream_t stream) {
+  spoa_hip_dev::ResetCounters<<<1, 256, 0, stream>>>(counters, n);
+  return HipOk(hipGetLastError(), "counter reset");
 }
 
-KParams MakeParams(const spoa_hip_params& p, int width_bits, int lanes, int k) {
-  KParams kp{};
-  kp.type = p.type;
-  kp.m = p.m;
-  kp.mis = p.n;
-  kp.g = p.g;
-  kp.e = p.e;
-  kp.q = p.q;
-  kp.c = p.c;
-  // reference simd...:526-528
-  kp.pad = -1 * std::max(std::max(std::abs(static_cast<int>(p.m)), std::abs(static_cast<int>(p.n))),
-                         std::max(std::abs(static_cast<int>(p.g)), std::abs(static_cast<int>(p.q))));
-  kp.ninf = width_bits == 16 ? std::numeric_limits<int16_t>::min() + 1024
-                             : std::numeric_limits<int32_t>::min() + 1024;
-  kp.decay = p.subtype == 0 ? k * p.g : k * p.e;
-  if (p.subtype == 2) {
-    // one L-column segment: (E, Q) entering the segment -> entering the next
-    const int L = lanes;
-    const int32_t seg[4] = {(L - 1) * p.e + std::max<int32_t>(p.g, p.e), (L - 1) * p.c + p.g,
-                            (L - 1) * p.e + p.q, (L - 1) * p.c + std::max<int32_t>(p.q, p.c)};
-    std::memcpy(kp.mseg, seg, sizeof(seg));
-    int32_t lane[4];
-    std::memcpy(lane, seg, sizeof(seg));
-    for (int s = 1; s < k / L; ++s) MpMul(seg, lane, lane);
-    std::memcpy(kp.mpow[0], lane, sizeof(lane));
-    for (int t = 1; t < 7; ++t) MpMul(kp.mpow[t - 1], kp.mpow[t - 1], kp.mpow[t]);
+// Places matrices of the given sizes in the engine's arena segments (first
+// fit, in order), growing the arena within the memory budget (at least
+// doubling it, so that batches whose jobs grow round by round allocate
+// O(log) times); when the segments are too fragmented, starts over with one
+// segment holding everything.  Device addresses into addr.
+int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::vector<uint64_t>* addr_out) {
+  const size_t nj = need.size();
+  std::vector<uint64_t>& addr = *addr_out;
+  addr.assign(nj, 0);
+  std::vector<uint64_t> used(E->arena.size(), 0);
+  size_t x = 0;
+  bool retried = false;
+  while (x < nj) {
+    bool placed = false;
+    for (size_t sgi = 0; sgi < E->arena.size(); ++sgi) {
+      if (used[sgi] + need[x] <= E->arena[sgi].size) {
+        addr[x] = reinterpret_cast<uint64_t>(E->arena[sgi].p) + used[sgi];
+        used[sgi] += need[x];
+        placed = true;
+        break;
+      }
+    }
+    if (placed) {
+      ++x;
+      continue;
+    }
+    uint64_t rest = 0;
+    for (size_t y = x; y < nj; ++y) rest += need[y];
+    const uint64_t room = E->budget > E->arena_bytes ? E->budget - E->arena_bytes : 0;
+    uint64_t want = std::min(room, std::max<uint64_t>(std::max<uint64_t>(rest, 4ull << 30), E->arena_bytes));
+    if (want < need[x]) {
+      if (retried) return Fail(SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+      retried = true;
+      for (auto& sg : E->arena) sg.Release();
+      E->arena.clear();
+      E->arena_bytes = 0;
+      used.clear();
+      x = 0;
+      continue;
+    }
+    DevBuf sg;
+    if (hipMalloc(&sg.p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      want = rest;
+      if (want < need[x] || hipMalloc(&sg.p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        return Fail(SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+      }
+    }
+    sg.size = want;
+    E->arena.push_back(sg);
+    E->arena_bytes += want;
+    used.push_back(0);
   }
-  return kp;
+  return SPOA_HIP_OK;
 }
+
+// Launch groups of a pass: plans[lo, hi) sorted by launch group, the largest
+// V * n first inside a group.  Per group: its kernel, workgroup size, LDS,
+// grid (workgroups that fit the device at once, at most one a job) and the
+// number of leading jobs that run at raised wave priority.
+int BuildGroups(spoa_hip_engine* E, const std::vector<Plan>& plans, size_t lo, size_t hi,
+                std::vector<LaunchGroup>* out) {
+  const spoa_hip_params& P = E->prm;
+  std::vector<LaunchGroup>& groups = *out;
+  groups.clear();
+  for (size_t x = lo; x < hi;) {
+    size_t y = x;
+    while (y < hi && SameGroup(E->prm.subtype, plans[y], plans[x])) ++y;
+    LaunchGroup gp;
+    gp.a = x - lo;
+    gp.b = y - lo;
+    gp.fn = GroupKernel(E, plans[x]);
+    gp.threads = 64 * plans[x].cfg.waves;
+    gp.lds = 8u * static_cast<uint32_t>(plans[x].cfg.k) * static_cast<uint32_t>(gp.threads);
+    if (!gp.fn) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] unsupported launch configuration");
+    auto blocks_per_cu = [&](uint32_t lds, int* n_out) -> bool {
+      auto key = std::make_tuple(gp.fn, gp.threads, lds);
+      auto oc = E->occupancy.find(key);
+      if (oc != E->occupancy.end()) {
+        *n_out = oc->second;
+        return true;
+      }
+      if (!HipOk(hipOccupancyMaxActiveBlocksPerMultiprocessor(n_out, gp.fn, gp.threads, lds), "occupancy")) {
+        return false;
+      }
+      E->occupancy[key] = *n_out;
+      return true;
+    };
+    int per_cu = 0;
+    if (!blocks_per_cu(gp.lds, &per_cu)) return SPOA_HIP_DEVICE_ERROR;
+    const uint32_t n = static_cast<uint32_t>(gp.b - gp.a);
+    // Workgroups take the LDS of their query profile (the traceback's tile
+    // lives in it), enlarged to 24 or 32 KiB only where that leaves the
+    // kernel's own workgroups per CU (set by its registers) unchanged:
+    // launches of the two halves and of several launch groups share the
+    // device, and workgroups enlarged to fill it alone kept the other
+    // launches' workgroups waiting (11% more windows/s on 2,048 local
+    // convex 1.5 kb windows without that enlargement; about 1% more with
+    // this one).
+    for (uint32_t cand : {32768u, 24576u}) {
+      if (cand <= gp.lds) break;
+      int pc = 0;
+      if (!blocks_per_cu(cand, &pc)) return SPOA_HIP_DEVICE_ERROR;
+      if (pc >= per_cu) {
+        gp.lds = cand;
+        break;
+      }
+    }
+    gp.grid = std::max<uint32_t>(
+        1u, std::min<uint32_t>(n, static_cast<uint32_t>(std::max(1, per_cu)) * static_cast<uint32_t>(E->num_cus)));
+    gp.prm = MakeParams(P, plans[x].bits, plans[x].lanes, plans[x].cfg.k);
+    // the group's jobs of at least half its largest job's V * n (they lead
+    // the queue) get the raised wave priority (1.7% less kernel time on
+    // 2,048 local convex 1.5 kb windows; raising the side launches' jobs
+    // above them was slower)
+    const uint64_t top = plans[x].work;
+    gp.prio = 0;
+    while (x + gp.prio < y && 2 * plans[x + gp.prio].work >= top) ++gp.prio;
+    groups.push_back(gp);
+    x = y;
+  }
+  return SPOA_HIP_OK;
+}
+
+}  // namespace spoa_hip_host
+
+namespace {
 
 // First columns (reference simd...:568-577, 590-599, 607-694): fcF / fcO
 // are the vertical-gap columns, fcH the primary column per alignment type.
@@ -486,115 +444,7 @@ __global__ void NoAllocEmpty(int32_t* result) {
 
 }  // namespace spoa_hip_dev
 
-struct Plan {
-  uint32_t job;
-  int width;  // the reference's score width
-  int bits;   // stored rows
-  int lanes;  // reference lane count (row width, convex segments)
-  Cfg cfg;
-  uint32_t W, stride;
-  uint64_t mat_bytes;
-  uint64_t graph_words;
-  uint64_t seq_bytes;
-};
-
-struct spoa_hip_engine {
-  int device = 0;
-  // host threads packing and unpacking large passes (made on first use)
-  std::unique_ptr<spoa_amd::ThreadPool> pool;
-  hipStream_t stream = nullptr;   // the engine's own stream
-  hipStream_t copy = nullptr;     // copies: the own stream, or a shared one
-  // kernels: the engine's own stream, or one shared with other engines so
-  // that their launches run one after another and never share the device
-  hipStream_t compute = nullptr;
-  hipStream_t compute_own = nullptr;  // created when other engines share this one's kernels
-  // launch groups after a pass's first run on this stream, beside the first
-  // (for example the few int32-width jobs of a pass beside its int16 jobs):
-  // the engine's own stream when its kernels run on a shared compute stream
-  // (its copies are idle while its pass runs), else a stream made on first
-  // use (streams map onto the device's few hardware queues in creation
-  // order, and engines that never need one must not shift that mapping for
-  // the streams made after them)
-  hipStream_t side = nullptr;
-  hipStream_t side_own = nullptr;
-  hipEvent_t ev_side_in = nullptr, ev_side = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;  // copy timing
-  hipEvent_t ev_in = nullptr, ev_kdone = nullptr;  // copy -> kernels -> copy ordering
-  std::vector<hipEvent_t> ev_ks, ev_ke;            // start / end of every launch of a pass
-  size_t pass_launches = 0;
-  // graph mode: every pass's launch sequence is captured into a HIP graph
-  // and launched as one unit on the compute stream
-  bool graph_mode = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  bool graph_pass = false;  // the pending pass ran as a graph
-  std::vector<int> pass_width;  // score width of every launch of the pass
-  spoa_hip_params prm{};
-  int num_cus = 256;
-  uint64_t budget = 0;
-  DevBuf d_in, d_out, d_counter;
-  // DP matrices: grow-only segments kept for the engine's lifetime, so a
-  // long-lived engine allocates once (allocation time grows with size)
-  std::vector<DevBuf> arena;
-  uint64_t arena_bytes = 0;
-  // a launched pass not yet finished (the asynchronous batch form)
-  struct Pending {
-    bool active = false;
-    spoa_hip_job* jobs = nullptr;
-    std::vector<Plan> plans;
-    size_t lo = 0, hi = 0;
-    uint64_t off_res = 0, off_pairs = 0;
-    std::vector<uint64_t> pair_at;
-    int first = 0;  // status of the jobs rejected before any launch
-  } pending;
-  PinBuf h_in, h_out;
-  spoa_hip_stats stats{};
-  std::map<std::tuple<KernelFn, int, uint32_t>, int> occupancy;  // (kernel, threads, LDS) -> blocks a CU
-  // overrides: (width, batch) -> max_width -> cfg
-  std::map<std::pair<int, int>, std::map<uint32_t, Cfg>> overrides;
-};
-
 namespace {
-
-// Batched launches: 16 columns a lane for int16 rows (at most 1,024
-// columns a wave), 8 for int32 rows (512 a wave), as many waves as the row
-// needs up to 8 (wider rows take several tiles).  Measured on batches of
-// 1.5 kb local convex and 500 bp overlap affine windows, this beats both
-// longer lanes (K = 32, one wave) and more waves of shorter lanes.  The per-call launch takes the widest workgroup
-// (up to 8 waves) of the shortest lanes.
-Cfg DefaultCfg(const spoa_hip_engine* E, uint32_t W, int width_bits, bool batch) {
-  Cfg c;
-  const bool convex16 = E->prm.subtype == 2 && width_bits == 16;
-  const int kmin = convex16 ? 16 : 8;
-  c.k = batch ? (width_bits == 16 ? 16 : 8) : kmin;
-  const uint32_t per_wave = 64u * static_cast<uint32_t>(c.k);
-  c.waves = static_cast<int>(
-      std::min<uint32_t>(kMaxGroupWaves, std::max<uint32_t>(1u, (W + per_wave - 1) / per_wave)));
-  return c;
-}
-
-Cfg ChooseCfg(const spoa_hip_engine* E, uint32_t len, int width_bits, bool batch) {
-  const int L = Lanes(width_bits);
-  const uint32_t W = (len + L - 1) / L * L;
-  auto it = E->overrides.find({width_bits, batch ? 1 : 0});
-  if (it != E->overrides.end()) {
-    auto jt = it->second.lower_bound(W);
-    if (jt != it->second.end()) return jt->second;
-  }
-  return DefaultCfg(E, W, width_bits, batch);
-}
-
-// Device bytes of one job: its DP matrices, plus -- when a row spans
-// several wave segments (waves x tiles) -- the H value left of every
-// segment for every row (int32), which the first lane of a segment keeps
-// for the diagonal moves of later rows.
-uint64_t JobBytes(const spoa_hip_params& p, uint32_t V, uint32_t stride, int width, const Cfg& cfg) {
-  uint64_t b = static_cast<uint64_t>(V + 1) * Matrices(p.subtype) * stride * (width / 8);
-  const uint64_t tile = 64ull * static_cast<uint64_t>(cfg.k) * static_cast<uint64_t>(cfg.waves);
-  const uint64_t nseg = (stride + tile - 1) / tile * static_cast<uint64_t>(cfg.waves);
-  if (nseg > 1) b += nseg * (V + 1) * 4ull;
-  return RoundUp(b, kAlign);
-}
 
 // Packs one job's graph block and read into the staging buffer.
 // *r_out: R = sum over the rows of max(1, predecessors), for the DP's
@@ -660,21 +510,6 @@ void ParallelRanges(spoa_hip_engine* E, size_t n, const Fn& fn) {
   E->pool->For((n + kChunk - 1) / kChunk, [&](size_t c) { fn(c * kChunk, std::min(n, (c + 1) * kChunk)); });
 }
 
-KernelFn GroupKernel(spoa_hip_engine* E, const Plan& pl) {
-  return PickKernel(E->prm.subtype, pl.bits, pl.lanes, pl.cfg.k, pl.cfg.waves);
-}
-
-// Launch group of a job: jobs of one group share a kernel and its
-// parameters.  Linear and affine rows depend on the lane count only through
-// the row width each job carries, so their int16 rows share one group.
-int GroupLanes(int subtype, const Plan& pl) { return subtype == 2 || pl.bits == 32 ? pl.lanes : 16; }
-
-
-bool SameGroup(int subtype, const Plan& a, const Plan& b) {
-  return a.bits == b.bits && GroupLanes(subtype, a) == GroupLanes(subtype, b) && a.cfg.k == b.cfg.k &&
-         a.cfg.waves == b.cfg.waves;
-}
-
 // One pass: the jobs plans[lo, hi) (sorted by launch group, largest first
 // inside a group) are packed into one staging block, copied in one
 // transfer on the engine's copy stream, aligned by one persistent launch
@@ -714,55 +549,10 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   // place every job's matrices in an arena segment (first fit in order)
   std::vector<uint64_t> addr(nj);
   {
-    std::vector<uint64_t> used(E->arena.size(), 0);
-    size_t x = 0;
-    bool retried = false;
-    while (x < nj) {
-      const uint64_t need = plans[lo + x].mat_bytes;
-      bool placed = false;
-      for (size_t sgi = 0; sgi < E->arena.size(); ++sgi) {
-        if (used[sgi] + need <= E->arena[sgi].size) {
-          addr[x] = reinterpret_cast<uint64_t>(E->arena[sgi].p) + used[sgi];
-          used[sgi] += need;
-          placed = true;
-          break;
-        }
-      }
-      if (placed) {
-        ++x;
-        continue;
-      }
-      uint64_t rest = 0;
-      for (size_t y = x; y < nj; ++y) rest += plans[lo + y].mat_bytes;
-      const uint64_t room = E->budget > E->arena_bytes ? E->budget - E->arena_bytes : 0;
-      // grow geometrically (at least doubling the arena) so that batches
-      // whose jobs grow round by round allocate O(log) times
-      uint64_t want = std::min(room, std::max<uint64_t>(std::max<uint64_t>(rest, 4ull << 30), E->arena_bytes));
-      if (want < need) {
-        // fragmented: start over with one segment holding the whole pass
-        if (retried) return Fail(SPOA_HIP_NO_MEMORY, kMsgNoMemory);
-        retried = true;
-        for (auto& sg : E->arena) sg.Release();
-        E->arena.clear();
-        E->arena_bytes = 0;
-        used.clear();
-        x = 0;
-        continue;
-      }
-      DevBuf sg;
-      if (hipMalloc(&sg.p, want) != hipSuccess) {
-        (void)hipGetLastError();
-        want = rest;
-        if (want < need || hipMalloc(&sg.p, want) != hipSuccess) {
-          (void)hipGetLastError();
-          return Fail(SPOA_HIP_NO_MEMORY, kMsgNoMemory);
-        }
-      }
-      sg.size = want;
-      E->arena.push_back(sg);
-      E->arena_bytes += want;
-      used.push_back(0);
-    }
+    std::vector<uint64_t> need(nj);
+    for (size_t x = 0; x < nj; ++x) need[x] = plans[lo + x].mat_bytes;
+    const int rc = PlaceMatrices(E, need, &addr);
+    if (rc != SPOA_HIP_OK) return rc;
   }
   E->stats.alloc_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_alloc0).count();
 
@@ -826,74 +616,10 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   E->stats.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
 
   // ---- launch groups
-  struct Group {
-    size_t a, b;  // job range inside the pass
-    KernelFn fn;
-    int threads;
-    uint32_t lds;  // query-profile bytes (4 bases x K/2 words a lane)
-    uint32_t grid;
-    uint32_t prio;  // leading jobs (largest V * n) at raised wave priority
-    KParams prm;
-  };
-  std::vector<Group> groups;
-  for (size_t x = lo; x < hi;) {
-    size_t y = x;
-    while (y < hi && SameGroup(E->prm.subtype, plans[y], plans[x])) ++y;
-    Group gp;
-    gp.a = x - lo;
-    gp.b = y - lo;
-    gp.fn = GroupKernel(E, plans[x]);
-    gp.threads = 64 * plans[x].cfg.waves;
-    gp.lds = 8u * static_cast<uint32_t>(plans[x].cfg.k) * static_cast<uint32_t>(gp.threads);
-    if (!gp.fn) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] unsupported launch configuration");
-    auto blocks_per_cu = [&](uint32_t lds, int* out) -> bool {
-      auto key = std::make_tuple(gp.fn, gp.threads, lds);
-      auto oc = E->occupancy.find(key);
-      if (oc != E->occupancy.end()) {
-        *out = oc->second;
-        return true;
-      }
-      if (!HipOk(hipOccupancyMaxActiveBlocksPerMultiprocessor(out, gp.fn, gp.threads, lds), "occupancy")) return false;
-      E->occupancy[key] = *out;
-      return true;
-    };
-    int per_cu = 0;
-    if (!blocks_per_cu(gp.lds, &per_cu)) return SPOA_HIP_DEVICE_ERROR;
-    const uint32_t n = static_cast<uint32_t>(gp.b - gp.a);
-    // Workgroups take the LDS of their query profile (the traceback's tile
-    // lives in it), enlarged to 24 or 32 KiB only where that leaves the
-    // kernel's own workgroups per CU (set by its registers) unchanged:
-    // launches of the two halves and of several launch groups share the
-    // device, and workgroups enlarged to fill it alone kept the other
-    // launches' workgroups waiting (11% more windows/s on 2,048 local
-    // convex 1.5 kb windows without that enlargement; about 1% more with
-    // this one).
-    for (uint32_t cand : {32768u, 24576u}) {
-      if (cand <= gp.lds) break;
-      int pc = 0;
-      if (!blocks_per_cu(cand, &pc)) return SPOA_HIP_DEVICE_ERROR;
-      if (pc >= per_cu) {
-        gp.lds = cand;
-        break;
-      }
-    }
-    gp.grid = std::max<uint32_t>(
-        1u, std::min<uint32_t>(n, static_cast<uint32_t>(std::max(1, per_cu)) * static_cast<uint32_t>(E->num_cus)));
-    gp.prm = MakeParams(P, plans[x].bits, plans[x].lanes, plans[x].cfg.k);
-    // the group's jobs of at least half its largest job's V * n (they lead
-    // the queue) get the raised wave priority (1.7% less kernel time on
-    // 2,048 local convex 1.5 kb windows; raising the side launches' jobs
-    // above them was slower)
-    {
-      auto work = [&](size_t z) {
-        return static_cast<uint64_t>(jobs[plans[z].job].graph.num_nodes) * jobs[plans[z].job].len;
-      };
-      const uint64_t top = work(x);
-      gp.prio = 0;
-      while (x + gp.prio < y && 2 * work(x + gp.prio) >= top) ++gp.prio;
-    }
-    groups.push_back(gp);
-    x = y;
+  std::vector<LaunchGroup> groups;
+  {
+    const int rc = BuildGroups(E, plans, lo, hi, &groups);
+    if (rc != SPOA_HIP_OK) return rc;
   }
   if (groups.size() > 256) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] too many launch groups");
 
@@ -914,7 +640,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   hipStream_t st = E->copy;
   hipStream_t cs = E->compute;
   auto args_of = [&](size_t gi) {
-    const Group& gp = groups[gi];
+    const LaunchGroup& gp = groups[gi];
     KArgs args{};
     args.prm = gp.prm;
     args.jobs = reinterpret_cast<const DevJob*>(din + off_jobs) + gp.a;
@@ -931,7 +657,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   E->pass_width.clear();
   // statistics by the reference's width (its int32 jobs on int16 rows count
   // with the int32 launches, whose launch shape they take)
-  for (const Group& gp : groups) E->pass_width.push_back(plans[lo + gp.a].width);
+  for (const LaunchGroup& gp : groups) E->pass_width.push_back(plans[lo + gp.a].width);
   E->stats.launches += static_cast<uint32_t>(groups.size());
   E->graph_pass = E->graph_mode;
   if (E->graph_mode) {
@@ -947,7 +673,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
       ok = HipOk(hipGetLastError(), "counter reset");
     }
     for (size_t gi = 0; ok && gi < groups.size(); ++gi) {
-      const Group& gp = groups[gi];
+      const LaunchGroup& gp = groups[gi];
       gp.fn<<<gp.grid, gp.threads, gp.lds, st>>>(args_of(gi));
       ok = HipOk(hipGetLastError(), "kernel launch");
     }
@@ -1000,7 +726,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
       if (!HipOk(hipGetLastError(), "counter reset")) return SPOA_HIP_DEVICE_ERROR;
     }
     for (size_t gi = 0; gi < groups.size(); ++gi) {
-      const Group& gp = groups[gi];
+      const LaunchGroup& gp = groups[gi];
       hipStream_t ks = gi > 0 && beside ? E->side : cs;
       if (!HipOk(hipEventRecord(E->ev_ks[gi], ks), "event")) return SPOA_HIP_DEVICE_ERROR;
       gp.fn<<<gp.grid, gp.threads, gp.lds, ks>>>(args_of(gi));
@@ -1189,6 +915,7 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     // reference's int32 jobs take the int32 launch shape: they form the
     // same few, long launches)
     pl.cfg = ChooseCfg(E, batch ? cfg_len[sh.width == 32] : j.len, sh.width, batch);
+    pl.work = static_cast<uint64_t>(V) * j.len;
     const int L = sh.lanes;
     pl.W = (j.len + L - 1) / L * L;
     pl.stride = static_cast<uint32_t>(RoundUp(pl.W, static_cast<uint64_t>(pl.cfg.k)));
