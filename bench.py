@@ -301,6 +301,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch configurations)")
     ap.add_argument("--tune-rounds", type=int, default=8)
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (JSON)")
+    ap.add_argument("--merge", choices=["auto", "host", "device"], default="auto",
+                    help="where window graphs live and are merged: host threads, or device-resident "
+                         "(auto: host for the 1.5 kb local convex windows, whose 9k-node graphs sort faster "
+                         "on 16 host threads beside the alignments; device for the others)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -328,7 +332,8 @@ def main():
     strand = cfg.get("strand", False)
     windows = spoa_amd.config_windows(args.config, args.seed, n_windows)
 
-    ctx = spoa_amd.BatchContext(*params, strand=strand, msa=True, threads=threads)
+    merge = args.merge if args.merge != "auto" else ("host" if args.config == 2 else "device")
+    ctx = spoa_amd.BatchContext(*params, strand=strand, msa=True, threads=threads, resident=merge == "device")
     tuned = None
     if not args.no_tune:
         # launch configuration per score width from the autotuner (untimed)
@@ -387,7 +392,8 @@ def main():
         "dtype": "int16/int32",
         "data": "synthetic (seeded uniform ACGT templates, mixed substitution/insertion/deletion errors)",
         "config": {"workload": cfg["name"], "windows_per_step": n_windows, "seed": args.seed,
-                   "params": list(params), "strand": strand, "host_threads": threads},
+                   "params": list(params), "strand": strand, "host_threads": threads,
+                   "graphs": "device-resident" if merge == "device" else "host"},
         "gcups_dp": cells / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else 0.0,
         "gcups_end_to_end": cells / el / 1e9,
         "roofline": dict(roof, traffic=measured_traffic(cfg["name"], n_windows),

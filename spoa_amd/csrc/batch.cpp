@@ -240,6 +240,7 @@ std::vector<WindowResult> RunResident(const WindowSet& ws, BatchContext& ctx, Ba
     st.dp_bytes += hs.dp_bytes;
     st.align_calls += hs.jobs;
     st.pack_ms += hs.host_ms;
+    st.alloc_ms += hs.alloc_ms;
     st.graph_ms += hs.merge_ms;  // device time of the merges
     spoa_hip_windows_destroy(p.win);
     p.win = nullptr;

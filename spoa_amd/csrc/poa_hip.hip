@@ -108,12 +108,27 @@ int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::ve
     const uint64_t room = E->budget > E->arena_bytes ? E->budget - E->arena_bytes : 0;
     uint64_t want = std::min(room, std::max<uint64_t>(std::max<uint64_t>(rest, 4ull << 30), E->arena_bytes));
     if (want < need[x]) {
+      // fragmented: start over with one segment of twice what the whole
+      // set needs (within the budget), so that sets which grow round by
+      // round do not free and allocate the arena every round
       if (retried) return Fail(SPOA_HIP_NO_MEMORY, kMsgNoMemory);
       retried = true;
       for (auto& sg : E->arena) sg.Release();
       E->arena.clear();
       E->arena_bytes = 0;
       used.clear();
+      uint64_t all = 0;
+      for (uint64_t b : need) all += b;
+      DevBuf sg;
+      const uint64_t one = std::min<uint64_t>(E->budget, std::max<uint64_t>(2 * all, 4ull << 30));
+      if (one >= all && hipMalloc(&sg.p, one) == hipSuccess) {
+        sg.size = one;
+        E->arena.push_back(sg);
+        E->arena_bytes += one;
+        used.push_back(0);
+      } else {
+        (void)hipGetLastError();
+      }
       x = 0;
       continue;
     }

@@ -592,6 +592,7 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
     pair_at[x + 1] = pair_at[x] + X->state[jr.window].V + X->read_len[X->read_first[jr.window] + r];
   }
   if (pair_at[nj] > 0xffffffffull) return WinFail(X, SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+  const auto t_alloc0 = std::chrono::steady_clock::now();
   const uint64_t off_items = RoundUp(nj * sizeof(DevJob), kAlign);
   const uint64_t table_bytes = RoundUp(off_items + X->live.size() * sizeof(MergeItem), kAlign) + kAlign;
   if (!X->h_table.Reserve(table_bytes) || !X->d_table.Reserve(table_bytes, kMsgNoMemory) ||
@@ -645,6 +646,7 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
       X->stats.jobs += 1;
     }
   }
+  X->stats.alloc_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_alloc0).count();
   // merge items (job indices by window)
   {
     std::vector<int32_t> jf(X->n, -1), jr(X->n, -1);
