@@ -148,6 +148,31 @@ def cpu_baseline(windows, params, strand, budget_s, threads):
             "consensus": results}
 
 
+def copy_probe(nbytes):
+    """One plain device-to-device copy of nbytes through the HIP runtime the
+    library itself loads (its counters calibrate HBM byte counts).  No other
+    GPU runtime may share the process: torch, for one, ships a HIP runtime
+    of its own under the same soname, and whichever loads first serves
+    both."""
+    spoa_amd.lib()
+    rt = ctypes.CDLL("libamdhip64.so.7")
+    rt.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    rt.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    rt.hipFree.argtypes = [ctypes.c_void_p]
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if rt.hipMalloc(ctypes.byref(src), nbytes) != 0 or rt.hipMalloc(ctypes.byref(dst), nbytes) != 0:
+        raise RuntimeError("copy probe: hipMalloc failed")
+    try:
+        if rt.hipMemset(src, 1, nbytes) != 0 or rt.hipMemcpy(dst, src, nbytes, 3) != 0:  # 3: device to device
+            raise RuntimeError("copy probe: copy failed")
+        if rt.hipDeviceSynchronize() != 0:
+            raise RuntimeError("copy probe: synchronize failed")
+    finally:
+        rt.hipFree(src)
+        rt.hipFree(dst)
+
+
 def pct(xs, p):
     xs = sorted(xs)
     if not xs:
@@ -315,12 +340,7 @@ def main():
         sys.exit(2)
 
     if args.copy_probe:
-        import torch
-        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        copy_probe(args.copy_probe << 20)
 
     threads = args.threads or min(16, os.cpu_count() or 1)
     if args.config == 5:

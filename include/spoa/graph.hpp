@@ -1,8 +1,8 @@
 // spoa::Graph -- drop-in for the reference POA graph
 // (include/spoa/graph.hpp:26-318 of spoa v4.0.8): the same public node and
 // edge records, accessors, AddAlignment overloads, consensus and MSA.
-// Out of the hot path and not provided: Subgraph / UpdateAlignment,
-// PrintDot and the coverage-summary GenerateConsensus overload.
+// Also the windowed-alignment helpers (Subgraph, UpdateAlignment), the
+// Graphviz writer (PrintDot) and the coverage-summary GenerateConsensus.
 //
 // Beyond the reference, the graph keeps a mutation counter and a
 // rank-ordered CSR view (predecessor ranks in in-edge order, decoded bases,
@@ -89,6 +89,15 @@ class Graph {
 
   std::string GenerateConsensus();
 
+  std::string GenerateConsensus(std::vector<std::uint32_t>* summary, bool verbose = false);
+
+  Graph Subgraph(std::uint32_t begin, std::uint32_t end, std::vector<const Node*>* subgraph_to_graph) const;
+
+  void UpdateAlignment(const std::vector<const Node*>& subgraph_to_graph, Alignment* alignment) const;
+
+  // print with Graphviz
+  void PrintDot(const std::string& path) const;
+
   void Clear();
 
   // Rank-ordered CSR view used by the GPU engine (valid until the next
@@ -116,6 +125,7 @@ class Graph {
                          std::vector<Node*>* predecessors);
   std::vector<std::uint32_t> InitializeMultipleSequenceAlignment(
       std::uint32_t* row_size = nullptr) const;
+  std::vector<bool> ExtractSubgraph(const Node* from, const Node* floor) const;
 
   std::uint32_t num_codes_;
   std::vector<std::int32_t> coder_;

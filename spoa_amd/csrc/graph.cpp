@@ -7,6 +7,7 @@
 #include "spoa/graph.hpp"
 
 #include <algorithm>
+#include <fstream>
 #include <stdexcept>
 #include <unordered_set>
 
@@ -360,6 +361,138 @@ Graph::Node* Graph::BranchCompletion(std::uint32_t rank, std::vector<std::int64_
     if (!best || score[best->id] < score[nd->id]) best = nd;
   }
   return best;
+}
+
+// src/graph.cpp:377-431: the consensus and, per consensus position, its
+// coverage -- the reads through the node and its aligned nodes, or with
+// verbose one count per code (and a last row of gaps) of the reads whose
+// MSA column meets that position.
+std::string Graph::GenerateConsensus(std::vector<std::uint32_t>* summary, bool verbose) {
+  if (!summary) {
+    throw std::invalid_argument("[spoa::Graph::GenerateConsensus] error: invalid ptr to summary");
+  }
+  std::string dst = GenerateConsensus();
+  summary->clear();
+  const std::size_t C = consensus_.size();
+  if (!verbose) {
+    for (const Node* nd : consensus_) {
+      std::uint32_t cov = nd->Coverage();
+      for (const Node* a : nd->aligned_nodes) cov += a->Coverage();
+      summary->push_back(cov);
+    }
+    return dst;
+  }
+  summary->assign((num_codes_ + 1) * C, 0);
+  const std::vector<std::uint32_t> col = InitializeMultipleSequenceAlignment();
+  for (std::uint32_t s = 0; s < sequences_.size(); ++s) {
+    const Node* nd = sequences_[s];
+    std::uint32_t c = 0, last = 0, column = col[nd->id];
+    bool seen = false;  // an earlier node of this read met the consensus
+    while (true) {
+      // the first consensus position at or after the read's column
+      while (c < C && col[consensus_[c]->id] < column) ++c;
+      if (c < C && col[consensus_[c]->id] == column) {
+        if (seen) {
+          for (std::uint32_t j = last + 1; j < c; ++j) ++(*summary)[num_codes_ * C + j];
+        }
+        seen = true;
+        last = c;
+        ++(*summary)[nd->code * C + c];
+      }
+      if (c == C || !(nd = nd->Successor(s))) break;
+      column = col[nd->id];
+    }
+  }
+  return dst;
+}
+
+// src/graph.cpp:518-539: the nodes reached backwards from `from` (in-edges
+// and aligned nodes) without going below node id `floor`.
+std::vector<bool> Graph::ExtractSubgraph(const Node* from, const Node* floor) const {
+  std::vector<bool> in(nodes_.size(), false);
+  std::vector<const Node*> todo{from};
+  while (!todo.empty()) {
+    const Node* nd = todo.back();
+    todo.pop_back();
+    if (in[nd->id] || nd->id < floor->id) continue;
+    for (const Edge* e : nd->inedges) todo.push_back(e->tail);
+    for (const Node* a : nd->aligned_nodes) todo.push_back(a);
+    in[nd->id] = true;
+  }
+  return in;
+}
+
+// src/graph.cpp:541-595: the part of the graph between nodes begin and end
+// (everything end reaches backwards, down to id begin) as a graph of its
+// own with the same codes; subgraph_to_graph maps its node ids to this
+// graph's nodes.  Edges keep their weights (as 32-bit additions, as the
+// reference's AddEdge takes them) and carry no sequence labels.
+Graph Graph::Subgraph(std::uint32_t begin, std::uint32_t end, std::vector<const Node*>* subgraph_to_graph) const {
+  if (!subgraph_to_graph) {
+    throw std::invalid_argument("[spoa::Graph::Subgraph] error: invalid ptr to subgraph_to_graph");
+  }
+  if (begin >= nodes_.size() || end >= nodes_.size()) {
+    throw std::invalid_argument("[spoa::Graph::Subgraph] error: node out of range");
+  }
+  const std::vector<bool> in = ExtractSubgraph(nodes_[end].get(), nodes_[begin].get());
+  Graph sub;
+  sub.num_codes_ = num_codes_;
+  sub.coder_ = coder_;
+  sub.decoder_ = decoder_;
+  subgraph_to_graph->assign(nodes_.size(), nullptr);
+  std::vector<Node*> to_sub(nodes_.size(), nullptr);
+  for (const auto& nd : nodes_) {
+    if (!in[nd->id]) continue;
+    Node* s = sub.AddNode(nd->code);
+    to_sub[nd->id] = s;
+    (*subgraph_to_graph)[s->id] = nd.get();
+  }
+  for (const auto& nd : nodes_) {
+    if (!in[nd->id]) continue;
+    Node* s = to_sub[nd->id];
+    for (const Edge* e : nd->inedges) {
+      if (to_sub[e->tail->id]) sub.AddEdge(to_sub[e->tail->id], s, static_cast<std::uint32_t>(e->weight));
+    }
+    for (const Node* a : nd->aligned_nodes) {
+      if (to_sub[a->id]) s->aligned_nodes.push_back(to_sub[a->id]);
+    }
+  }
+  sub.TopologicalSort();
+  return sub;
+}
+
+// src/graph.cpp:597-605: node ids of an alignment against a subgraph back to
+// this graph's ids.
+void Graph::UpdateAlignment(const std::vector<const Node*>& subgraph_to_graph, Alignment* alignment) const {
+  for (auto& pr : *alignment) {
+    if (pr.first != -1) pr.first = static_cast<std::int32_t>(subgraph_to_graph[pr.first]->id);
+  }
+}
+
+// src/graph.cpp:607-647: Graphviz text of the graph -- nodes labelled
+// "id - base", out-edges labelled with their weights, aligned pairs as
+// dotted undirected lines, the last consensus highlighted.
+void Graph::PrintDot(const std::string& path) const {
+  if (path.empty()) return;
+  std::ofstream os(path);
+  std::vector<std::int32_t> on_consensus(nodes_.size(), -1);
+  std::int32_t k = 0;
+  for (const Node* nd : consensus_) on_consensus[nd->id] = k++;
+  os << "digraph " << sequences_.size() << " {" << std::endl << "  graph [rankdir = LR]" << std::endl;
+  for (const auto& nd : nodes_) {
+    os << "  " << nd->id << "[label = \"" << nd->id << " - " << static_cast<char>(decoder_[nd->code]) << "\"";
+    if (on_consensus[nd->id] != -1) os << ", style = filled, fillcolor = goldenrod1";
+    os << "]" << std::endl;
+    for (const Edge* e : nd->outedges) {
+      os << "  " << nd->id << " -> " << e->head->id << " [label = \"" << e->weight << "\"";
+      if (on_consensus[nd->id] + 1 == on_consensus[e->head->id]) os << ", color = goldenrod1";
+      os << "]" << std::endl;
+    }
+    for (const Node* a : nd->aligned_nodes) {
+      if (a->id > nd->id) os << "  " << nd->id << " -> " << a->id << " [style = dotted, arrowhead = none]" << std::endl;
+    }
+  }
+  os << "}" << std::endl;
 }
 
 // src/graph.cpp:649-658

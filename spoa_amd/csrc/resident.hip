@@ -609,6 +609,43 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
     passes.push_back({lo, hi});
     lo = hi;
   }
+  // Every pass places its matrices from the start of one arena segment
+  // large enough for the largest pass (passes run one after another on the
+  // stream).  The placements of all passes are made before any of them
+  // launches, so no placement may free a segment: when none is large
+  // enough, the arena is rebuilt here, while the stream is idle (the
+  // previous round has been waited for).
+  {
+    uint64_t largest_pass = 0;
+    for (const auto& ps : passes) {
+      uint64_t b = 0;
+      for (size_t x = ps.first; x < ps.second; ++x) b += plans[x].mat_bytes;
+      largest_pass = std::max(largest_pass, b);
+    }
+    uint64_t largest_seg = 0;
+    for (const auto& sg : E->arena) largest_seg = std::max(largest_seg, sg.size);
+    if (largest_pass > largest_seg) {
+      if (E->pending.active || !HipOk(hipStreamSynchronize(E->stream), "hipStreamSynchronize")) {
+        return WinFail(X, SPOA_HIP_DEVICE_ERROR, "[spoa::hip] the engine's stream is busy");
+      }
+      for (auto& sg : E->arena) sg.Release();
+      E->arena.clear();
+      E->arena_bytes = 0;
+      DevBuf sg;
+      uint64_t want = std::min<uint64_t>(E->budget, std::max<uint64_t>(2 * largest_pass, 4ull << 30));
+      if (hipMalloc(&sg.p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        want = largest_pass;
+        if (hipMalloc(&sg.p, want) != hipSuccess) {
+          (void)hipGetLastError();
+          return WinFail(X, SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+        }
+      }
+      sg.size = want;
+      E->arena.push_back(sg);
+      E->arena_bytes = want;
+    }
+  }
   DevJob* dj = X->h_table.At<DevJob>(0);
   std::vector<int32_t> job_of(refs.size(), -1);  // ref -> table index
   std::vector<std::vector<uint64_t>> pass_addr(passes.size());
