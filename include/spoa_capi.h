@@ -54,6 +54,31 @@ int spoa_graph_msa(void* graph, int include_consensus, char* out, uint64_t cap, 
 /* rank_to_node() ids (graph.hpp:110-112) */
 int spoa_graph_ranks(void* graph, uint32_t* out, uint32_t cap);
 
+/* Graph::Subgraph (reference include/spoa/graph.hpp:173-176): the part of
+ * the graph node `end` reaches backwards down to node id `begin`, as a new
+ * graph handle; sub_to_graph[k] (cap >= the graph's node count) gets the
+ * graph node id of subgraph node k, -1 past the subgraph's nodes.  NULL on
+ * error. */
+void* spoa_graph_subgraph(void* graph, uint32_t begin, uint32_t end, int32_t* sub_to_graph, uint32_t cap);
+/* Graph::UpdateAlignment (graph.hpp:178-180): node ids of n_pairs pairs
+ * aligned against a subgraph mapped back to the graph, in place. */
+int spoa_graph_update_alignment(void* graph, const int32_t* sub_to_graph, uint32_t n_map, int32_t* pairs,
+                                uint32_t n_pairs);
+/* Graph::PrintDot (graph.hpp:183): Graphviz text to the file at path. */
+int spoa_graph_print_dot(void* graph, const char* path);
+/* Graph::GenerateConsensus(summary, verbose) (graph.hpp:169-171): writes
+ * the consensus (returns its length) and the coverage summary
+ * (*summary_len values; -2 when a buffer is too small). */
+int spoa_graph_consensus_summary(void* graph, int verbose, char* out, uint32_t cap, uint32_t* summary,
+                                 uint32_t summary_cap, uint32_t* summary_len);
+/* The command-line tool's GFA output (reference src/main.cpp:119-199)
+ * after GenerateConsensus: headers holds one NUL-terminated name per
+ * sequence back to back; reversed (n_reversed flags or NULL) marks reads
+ * added reverse-complemented.  Returns the text length, or -2 - length
+ * when cap is too small, -1 on error. */
+int64_t spoa_graph_gfa(void* graph, const char* headers, const uint8_t* reversed, uint32_t n_reversed,
+                       int include_consensus, char* out, uint64_t cap);
+
 /* The flat partial-order graph of the batched form: the same operations
  * and results as spoa::Graph (AddAlignment, GenerateConsensus,
  * GenerateMultipleSequenceAlignment, rank_to_node). */

@@ -164,6 +164,105 @@ int ref_reverse_complement(const char* seq, std::uint32_t len, char* out) {
   return static_cast<int>(len);
 }
 
+// Graph::Subgraph through the reference (include/spoa/graph.hpp:173-176).
+void* ref_graph_subgraph(void* graph, std::uint32_t begin, std::uint32_t end, std::int32_t* sub_to_graph,
+                         std::uint32_t cap) {
+  try {
+    const auto& g = *static_cast<spoa::Graph*>(graph);
+    if (cap < g.nodes().size()) return nullptr;
+    std::vector<const spoa::Graph::Node*> map;
+    auto* sub = new spoa::Graph(g.Subgraph(begin, end, &map));
+    for (std::size_t k = 0; k < map.size(); ++k) sub_to_graph[k] = map[k] ? static_cast<std::int32_t>(map[k]->id) : -1;
+    return sub;
+  } catch (std::exception& ex) {
+    g_error = ex.what();
+    return nullptr;
+  }
+}
+
+// Graph::UpdateAlignment through the reference (graph.hpp:178-180).
+int ref_graph_update_alignment(void* graph, const std::int32_t* sub_to_graph, std::uint32_t n_map,
+                               std::int32_t* pairs, std::uint32_t n_pairs) {
+  const auto& g = *static_cast<spoa::Graph*>(graph);
+  std::vector<const spoa::Graph::Node*> map(n_map, nullptr);
+  for (std::uint32_t k = 0; k < n_map; ++k) {
+    if (sub_to_graph[k] >= 0) map[k] = g.nodes()[sub_to_graph[k]].get();
+  }
+  spoa::Alignment a(n_pairs);
+  for (std::uint32_t k = 0; k < n_pairs; ++k) a[k] = {pairs[2 * k], pairs[2 * k + 1]};
+  g.UpdateAlignment(map, &a);
+  for (std::uint32_t k = 0; k < n_pairs; ++k) pairs[2 * k] = a[k].first;
+  return 0;
+}
+
+int ref_graph_print_dot(void* graph, const char* path) {
+  static_cast<spoa::Graph*>(graph)->PrintDot(path);
+  return 0;
+}
+
+int ref_graph_consensus_summary(void* graph, int verbose, char* out, std::uint32_t cap, std::uint32_t* summary,
+                                std::uint32_t summary_cap, std::uint32_t* summary_len) {
+  std::vector<std::uint32_t> sum;
+  const std::string s = static_cast<spoa::Graph*>(graph)->GenerateConsensus(&sum, verbose != 0);
+  *summary_len = static_cast<std::uint32_t>(sum.size());
+  if (s.size() > cap || sum.size() > summary_cap) return -2;
+  std::memcpy(out, s.data(), s.size());
+  if (!sum.empty()) std::memcpy(summary, sum.data(), sum.size() * sizeof(std::uint32_t));
+  return static_cast<int>(s.size());
+}
+
+// The GFA of the reference's command-line tool, restated (its printer lives
+// in src/main.cpp:119-199, which needs the un-vendored parsers to build):
+// segments, links with weights, one path per sequence, the consensus path.
+std::int64_t ref_graph_gfa(void* graph, const char* headers, const std::uint8_t* reversed, std::uint32_t n_reversed,
+                           int include_consensus, char* out, std::uint64_t cap) {
+  auto& g = *static_cast<spoa::Graph*>(graph);
+  g.GenerateConsensus();
+  std::vector<std::string> names;
+  const char* h = headers;
+  for (std::size_t k = 0; k < g.sequences().size(); ++k) {
+    names.emplace_back(h);
+    h += names.back().size() + 1;
+  }
+  std::vector<bool> cons(g.nodes().size(), false);
+  for (const auto* nd : g.consensus()) cons[nd->id] = true;
+  std::string t = "H\tVN:Z:1.0\n";
+  for (const auto& nd : g.nodes()) {
+    t += "S\t" + std::to_string(nd->id + 1) + "\t" + static_cast<char>(g.decoder(nd->code));
+    if (cons[nd->id]) t += "\tic:Z:true";
+    t += "\n";
+    for (const auto* e : nd->outedges) {
+      t += "L\t" + std::to_string(nd->id + 1) + "\t+\t" + std::to_string(e->head->id + 1) + "\t+\tOM\tew:f:" +
+           std::to_string(e->weight);
+      if (cons[nd->id] && cons[e->head->id]) t += "\tic:Z:true";
+      t += "\n";
+    }
+  }
+  for (std::uint32_t i = 0; i < g.sequences().size(); ++i) {
+    t += "P\t" + names[i] + "\t";
+    std::vector<std::uint32_t> path;
+    for (auto* nd = g.sequences()[i]; nd; nd = nd->Successor(i)) path.push_back(nd->id + 1);
+    const bool rev = reversed && i < n_reversed && reversed[i];
+    if (rev) std::reverse(path.begin(), path.end());
+    for (std::size_t j = 0; j < path.size(); ++j) {
+      if (j) t += ",";
+      t += std::to_string(path[j]) + (rev ? "-" : "+");
+    }
+    t += "\t*\n";
+  }
+  if (include_consensus) {
+    t += "P\tConsensus\t";
+    for (std::size_t j = 0; j < g.consensus().size(); ++j) {
+      if (j) t += ",";
+      t += std::to_string(g.consensus()[j]->id + 1) + "+";
+    }
+    t += "\t*\n";
+  }
+  if (t.size() > cap) return -2 - static_cast<std::int64_t>(t.size());
+  std::memcpy(out, t.data(), t.size());
+  return static_cast<std::int64_t>(t.size());
+}
+
 // One window through the reference: every read aligned and added in order
 // (strand mode: the read or its reverse complement, whichever aligns with
 // the higher score, forward on ties, as src/main.cpp:285-302 does), then

@@ -7,11 +7,13 @@
 
 #include <cstring>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "batch.hpp"
+#include "io.hpp"
 #include "serve.hpp"
 #include "poa_graph.hpp"
 #include "spoa/spoa.hpp"
@@ -176,6 +178,98 @@ int spoa_graph_ranks(void* graph, uint32_t* out, uint32_t cap) {
   if (r.size() > cap) return -2;
   for (std::size_t k = 0; k < r.size(); ++k) out[k] = r[k]->id;
   return static_cast<int>(r.size());
+}
+
+void* spoa_graph_subgraph(void* graph, uint32_t begin, uint32_t end, int32_t* sub_to_graph, uint32_t cap) {
+  try {
+    const spoa::Graph& g = *Gr(graph);
+    if (cap < g.nodes().size()) throw std::invalid_argument("[spoa::Graph::Subgraph] error: map buffer too small");
+    std::vector<const spoa::Graph::Node*> map;
+    auto* sub = new spoa::Graph(g.Subgraph(begin, end, &map));
+    for (std::size_t k = 0; k < map.size(); ++k) sub_to_graph[k] = map[k] ? static_cast<int32_t>(map[k]->id) : -1;
+    return sub;
+  } catch (const std::exception& ex) {
+    Error(ex);
+    return nullptr;
+  }
+}
+
+int spoa_graph_update_alignment(void* graph, const int32_t* sub_to_graph, uint32_t n_map, int32_t* pairs,
+                                uint32_t n_pairs) {
+  try {
+    const spoa::Graph& g = *Gr(graph);
+    std::vector<const spoa::Graph::Node*> map(n_map, nullptr);
+    for (uint32_t k = 0; k < n_map; ++k) {
+      if (sub_to_graph[k] >= 0) {
+        if (static_cast<std::size_t>(sub_to_graph[k]) >= g.nodes().size()) {
+          throw std::invalid_argument("[spoa::Graph::UpdateAlignment] error: node out of range");
+        }
+        map[k] = g.nodes()[sub_to_graph[k]].get();
+      }
+    }
+    spoa::Alignment a(n_pairs);
+    for (uint32_t k = 0; k < n_pairs; ++k) {
+      a[k] = {pairs[2 * k], pairs[2 * k + 1]};
+      if (a[k].first >= static_cast<int32_t>(n_map) || (a[k].first >= 0 && !map[a[k].first])) {
+        throw std::invalid_argument("[spoa::Graph::UpdateAlignment] error: node not in the subgraph");
+      }
+    }
+    g.UpdateAlignment(map, &a);
+    for (uint32_t k = 0; k < n_pairs; ++k) pairs[2 * k] = a[k].first;
+    return 0;
+  } catch (const std::exception& ex) {
+    return Error(ex);
+  }
+}
+
+int spoa_graph_print_dot(void* graph, const char* path) {
+  try {
+    Gr(graph)->PrintDot(path ? path : "");
+    return 0;
+  } catch (const std::exception& ex) {
+    return Error(ex);
+  }
+}
+
+int spoa_graph_consensus_summary(void* graph, int verbose, char* out, uint32_t cap, uint32_t* summary,
+                                 uint32_t summary_cap, uint32_t* summary_len) {
+  try {
+    std::vector<std::uint32_t> sum;
+    const std::string s = Gr(graph)->GenerateConsensus(&sum, verbose != 0);
+    *summary_len = static_cast<uint32_t>(sum.size());
+    if (s.size() > cap || sum.size() > summary_cap) return -2;
+    std::memcpy(out, s.data(), s.size());
+    std::memcpy(summary, sum.data(), sum.size() * sizeof(uint32_t));
+    return static_cast<int>(s.size());
+  } catch (const std::exception& ex) {
+    return Error(ex);
+  }
+}
+
+int64_t spoa_graph_gfa(void* graph, const char* headers, const uint8_t* reversed, uint32_t n_reversed,
+                       int include_consensus, char* out, uint64_t cap) {
+  try {
+    spoa::Graph& g = *Gr(graph);
+    std::vector<std::string> names;
+    const char* h = headers;
+    for (std::size_t k = 0; k < g.sequences().size(); ++k) {
+      names.emplace_back(h);
+      h += names.back().size() + 1;
+    }
+    std::vector<bool> rev(reversed ? n_reversed : 0);
+    for (std::size_t k = 0; k < rev.size(); ++k) rev[k] = reversed[k] != 0;
+    g.GenerateConsensus();
+    std::ostringstream os;
+    if (!spoa_amd::WriteGfa(os, g, names, rev, include_consensus != 0)) {
+      throw std::invalid_argument("[spoa::PrintGfa] error: missing header(s)");
+    }
+    const std::string text = os.str();
+    if (text.size() > cap) return -2 - static_cast<int64_t>(text.size());
+    std::memcpy(out, text.data(), text.size());
+    return static_cast<int64_t>(text.size());
+  } catch (const std::exception& ex) {
+    return Error(ex);
+  }
 }
 
 // ----------------------------------------------------- flat POA graph --
