@@ -24,6 +24,11 @@ const char* spoa_last_error(void);
 /* AlignmentEngine::Create(type, m, n, g, e, q, c) (reference
  * include/spoa/alignment_engine.hpp:49-55); the engine runs on the GPU. */
 void* spoa_engine_new(int type, int m, int n, int g, int e, int q, int c);
+/* An engine that stands in for the reference's scalar engine
+ * (SisdAlignmentEngine::Create, src/sisd_alignment_engine.cpp:16-27, what
+ * AlignmentEngine::Create returns from a reference built without SIMD):
+ * the same validation as spoa_engine_new, that engine's results and messages. */
+void* spoa_sisd_engine_new(int type, int m, int n, int g, int e, int q, int c);
 void spoa_engine_free(void* engine);
 int spoa_engine_subtype(void* engine);
 /* AlignmentEngine::Prealloc (alignment_engine.hpp:57-59) */
@@ -103,6 +108,7 @@ typedef struct spoa_batch_opts {
   int32_t overlap; /* batched form: the two halves' launches may run on the device at once */
   int32_t graph;   /* batched form: every pass launched as a captured HIP graph */
   int32_t resident; /* batched form: window graphs on the device (merges, consensus and MSA there) */
+  int32_t sisd;     /* the reference's scalar engine's alignments (src/sisd_alignment_engine.cpp) */
 } spoa_batch_opts;
 
 typedef struct spoa_batch_stats {

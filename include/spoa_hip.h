@@ -80,6 +80,17 @@ typedef struct spoa_hip_engine spoa_hip_engine;
  * binds an engine with normalised parameters to a HIP device. */
 int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_engine** out);
 
+/* Replaces SisdAlignmentEngine::Create (reference
+ * src/sisd_alignment_engine.hpp:26-34, src/sisd_alignment_engine.cpp:16-27),
+ * the engine AlignmentEngine::Create returns from a reference built without
+ * SIMD (src/alignment_engine.cpp:68-71, e.g. its default x86 CMake flags):
+ * the same device path, with that engine's results and messages -- its
+ * overflow bound on (n, V), int32 scores (width_bits 32), full-Gotoh convex
+ * E / Q, its end cell (no padding columns, no 0 floor) and its traceback
+ * rules (src/sisd_alignment_engine.cpp:255-926).  Every entry point taking
+ * the engine follows them. */
+int spoa_hip_engine_create_sisd(int device, const spoa_hip_params* params, spoa_hip_engine** out);
+
 int spoa_hip_engine_destroy(spoa_hip_engine* engine);
 
 /* Replaces AlignmentEngine::Prealloc (reference alignment_engine.hpp:57-59,

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "spoa/spoa.hpp"
+#include "sisd_alignment_engine.hpp"  // src/: the scalar engine, reached as the reference library does
 
 namespace {
 thread_local std::string g_error;
@@ -64,6 +65,30 @@ const char* ref_last_error() { return g_error.c_str(); }
 void* ref_engine_new(int type, int m, int n, int g, int e, int q, int c) {
   try {
     return spoa::AlignmentEngine::Create(static_cast<spoa::AlignmentType>(type), m, n, g, e, q, c)
+        .release();
+  } catch (std::exception& ex) {
+    g_error = ex.what();
+    return nullptr;
+  }
+}
+
+// The reference's scalar engine (src/sisd_alignment_engine.cpp:16-27), the
+// one AlignmentEngine::Create returns when the library is built without
+// SIMD (src/alignment_engine.cpp:68-71, e.g. the default x86 CMake flags).
+// Create itself validates the parameters (same messages); the subtype
+// choice and normalisation below are those of src/alignment_engine.cpp:57-66.
+void* ref_sisd_engine_new(int type, int m, int n, int g, int e, int q, int c) {
+  try {
+    spoa::AlignmentEngine::Create(static_cast<spoa::AlignmentType>(type), m, n, g, e, q, c);
+    auto sub = g >= e ? spoa::AlignmentSubtype::kLinear
+                      : ((g <= q || e >= c) ? spoa::AlignmentSubtype::kAffine : spoa::AlignmentSubtype::kConvex);
+    if (sub == spoa::AlignmentSubtype::kLinear) {
+      e = g;
+    } else if (sub == spoa::AlignmentSubtype::kAffine) {
+      q = g;
+      c = e;
+    }
+    return spoa::SisdAlignmentEngine::Create(static_cast<spoa::AlignmentType>(type), sub, m, n, g, e, q, c)
         .release();
   } catch (std::exception& ex) {
     g_error = ex.what();

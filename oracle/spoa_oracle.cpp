@@ -15,6 +15,10 @@
 // the final H at vector boundaries
 // (src/simd_alignment_engine_implementation.hpp:1733-1769).
 //
+// SisdAlign restates the reference's scalar engine
+// (src/sisd_alignment_engine.cpp), selected per engine
+// (oracle_sisd_engine_new).
+//
 // Pinning: every function here is checked against the compiled reference
 // (oracle/_ref, built from /root/reference by oracle/Makefile) on seeded
 // random inputs and against the reference's 18 golden consensus strings
@@ -329,6 +333,7 @@ struct Graph {
 struct Engine {
   int type, subtype;
   std::int8_t m, n, g, e, q, c;
+  bool sisd = false;  // the scalar engine's semantics (SisdAlign)
 
   // src/alignment_engine.cpp:101-110
   std::int64_t WorstCase(std::int64_t i, std::int64_t j) const {
@@ -672,6 +677,287 @@ AlignResult Align(const Engine& en, const Graph& gr, const char* seq,
   return res;
 }
 
+// ---------------------------------------------------------- SISD engine --
+// The reference's scalar engine, SisdAlignmentEngine
+// (src/sisd_alignment_engine.cpp), which AlignmentEngine::Create returns
+// when the library is built without SIMD (src/alignment_engine.cpp:68-71).
+// Its DP is on (V+1) x (n+1) int32 matrices: column 0 is the first column,
+// row 0 the virtual source row.  Against the SIMD engine it differs in
+// three places: convex E/Q open from the final H of every column (full
+// Gotoh, :751-768), the end cell (no 0 floor, no padding columns,
+// :296-311 and :354-364), and the traceback (the convex left walk goes on
+// while either gap chain holds, :880-887; the NW walk runs through row 0
+// and column 0 in the main loop, not a separate tail).
+AlignResult SisdAlign(const Engine& en, const Graph& gr, const char* seq, std::uint32_t len) {
+  AlignResult res;
+  // :258-267
+  if (len > static_cast<std::uint32_t>(std::numeric_limits<std::int32_t>::max())) {
+    throw std::invalid_argument("[spoa::SisdAlignmentEngine::Align] error: too large sequence!");
+  }
+  const std::uint32_t V = gr.V();
+  if (V == 0 || len == 0) return res;
+  // :269-272: the bound on (n, V), not (n + 8, V); always int32
+  if (en.WorstCase(len, V) < std::numeric_limits<std::int32_t>::min() + 1024) {
+    throw std::invalid_argument("[spoa::SisdAlignmentEngine::Align] error: possible overflow!");
+  }
+  res.width_bits = 32;
+  const std::int32_t NINF = std::numeric_limits<std::int32_t>::min() + 1024;  // :13-14
+  const std::int64_t W = static_cast<std::int64_t>(len) + 1, R = V + 1;
+  const int T = en.type, S = en.subtype;
+  const std::int32_t m = en.m, n = en.n, g = en.g, e = en.e, q = en.q, c = en.c;
+
+  std::vector<std::uint32_t> rank_of(V, 0);
+  for (std::uint32_t r = 0; r < V; ++r) rank_of[gr.rank_to_node[r]] = r;
+  // predecessor rows in in-edge order; a source's only predecessor is row 0
+  std::vector<std::vector<std::int64_t>> in_rows(R), preds(R);
+  std::vector<bool> sink(R, false);
+  std::vector<std::int32_t> code(R, 0);
+  for (std::int64_t i = 1; i < R; ++i) {
+    const std::uint32_t id = gr.rank_to_node[i - 1];
+    for (auto ed : gr.in[id]) in_rows[i].push_back(rank_of[gr.tail[ed]] + 1);
+    preds[i] = in_rows[i];
+    if (preds[i].empty()) preds[i].push_back(0);
+    code[i] = static_cast<std::int32_t>(gr.code[id]);
+    sink[i] = gr.out[id].empty();
+  }
+  auto node_id = [&](std::int64_t row) { return static_cast<std::int32_t>(gr.rank_to_node[row - 1]); };
+  // :123-130 profile: column 0 scores 0, column j the match of seq[j - 1]
+  std::vector<std::int32_t> profile(static_cast<std::size_t>(gr.num_codes) * W, 0);
+  for (std::uint32_t k = 0; k < gr.num_codes; ++k) {
+    const char b = static_cast<char>(gr.decoder[k]);
+    for (std::int64_t j = 1; j < W; ++j) profile[k * W + j] = b == seq[j - 1] ? m : n;
+  }
+  auto prof = [&](std::int64_t i, std::int64_t j) { return profile[code[i] * W + j]; };
+
+  std::vector<std::int32_t> H(R * W, 0), F, E, O, Q;
+  if (S != kLinear) { F.assign(R * W, 0); E.assign(R * W, 0); }
+  if (S == kConvex) { O.assign(R * W, 0); Q.assign(R * W, 0); }
+  auto at = [&](std::vector<std::int32_t>& M, std::int64_t i, std::int64_t j) -> std::int32_t& { return M[i * W + j]; };
+
+  // :137-182 secondary matrices: row 0 and column 0
+  if (S == kConvex) {
+    O[0] = 0;
+    Q[0] = 0;
+    for (std::int64_t j = 1; j < W; ++j) { O[j] = NINF; Q[j] = q + static_cast<std::int32_t>(j - 1) * c; }
+    for (std::int64_t i = 1; i < R; ++i) {
+      std::int32_t pen = in_rows[i].empty() ? q - c : NINF;
+      for (auto p : in_rows[i]) pen = std::max(pen, at(O, p, 0));
+      at(O, i, 0) = pen + c;
+      at(Q, i, 0) = NINF;
+    }
+  }
+  if (S != kLinear) {
+    F[0] = 0;
+    E[0] = 0;
+    for (std::int64_t j = 1; j < W; ++j) { F[j] = NINF; E[j] = g + static_cast<std::int32_t>(j - 1) * e; }
+    for (std::int64_t i = 1; i < R; ++i) {
+      std::int32_t pen = in_rows[i].empty() ? g - e : NINF;
+      for (auto p : in_rows[i]) pen = std::max(pen, at(F, p, 0));
+      at(F, i, 0) = pen + e;
+      at(E, i, 0) = NINF;
+    }
+  }
+  H[0] = 0;
+  // :185-252 primary matrix: row 0 and column 0 per type
+  if (T == kSW) {
+    // all zero already
+  } else {
+    for (std::int64_t j = 1; j < W; ++j) {
+      if (S == kConvex) H[j] = std::max(Q[j], E[j]);
+      else if (S == kAffine) H[j] = E[j];
+      else H[j] = static_cast<std::int32_t>(j) * g;
+    }
+    for (std::int64_t i = 1; i < R; ++i) {
+      if (T == kOV) {
+        at(H, i, 0) = 0;
+      } else if (S == kConvex) {
+        at(H, i, 0) = std::max(at(O, i, 0), at(F, i, 0));
+      } else if (S == kAffine) {
+        at(H, i, 0) = at(F, i, 0);
+      } else {
+        std::int32_t pen = in_rows[i].empty() ? 0 : NINF;
+        for (auto p : in_rows[i]) pen = std::max(pen, at(H, p, 0));
+        at(H, i, 0) = pen + g;
+      }
+    }
+  }
+
+  // DP rows in rank order (:321-352 linear, :487-542 affine, :703-772 convex)
+  std::int32_t max_score = T == kSW ? 0 : NINF;
+  std::int64_t max_i = 0, max_j = 0;
+  for (std::int64_t i = 1; i < R; ++i) {
+    for (std::size_t k = 0; k < preds[i].size(); ++k) {
+      const std::int64_t p = preds[i][k];
+      for (std::int64_t j = 1; j < W; ++j) {
+        const std::int32_t dg = at(H, p, j - 1) + prof(i, j);
+        if (S == kLinear) {
+          const std::int32_t v = std::max(dg, at(H, p, j) + g);
+          at(H, i, j) = k == 0 ? v : std::max(at(H, i, j), v);
+        } else {
+          const std::int32_t fv = std::max(at(H, p, j) + g, at(F, p, j) + e);
+          at(F, i, j) = k == 0 ? fv : std::max(at(F, i, j), fv);
+          if (S == kConvex) {
+            const std::int32_t ov = std::max(at(H, p, j) + q, at(O, p, j) + c);
+            at(O, i, j) = k == 0 ? ov : std::max(at(O, i, j), ov);
+          }
+          at(H, i, j) = k == 0 ? dg : std::max(at(H, i, j), dg);
+        }
+      }
+    }
+    for (std::int64_t j = 1; j < W; ++j) {
+      // the horizontal gap opens from the final (clamped) H of column j - 1
+      std::int32_t h;
+      if (S == kLinear) {
+        h = std::max(at(H, i, j - 1) + g, at(H, i, j));
+      } else {
+        at(E, i, j) = std::max(at(H, i, j - 1) + g, at(E, i, j - 1) + e);
+        if (S == kAffine) {
+          h = std::max(at(H, i, j), std::max(at(F, i, j), at(E, i, j)));
+        } else {
+          at(Q, i, j) = std::max(at(H, i, j - 1) + q, at(Q, i, j - 1) + c);
+          h = std::max(at(H, i, j), std::max(std::max(at(F, i, j), at(E, i, j)), std::max(at(O, i, j), at(Q, i, j))));
+        }
+      }
+      if (T == kSW) h = std::max(h, 0);
+      at(H, i, j) = h;
+      // first strictly better cell in row-major order
+      const bool cand = T == kSW || (sink[i] && (T == kOV || j == W - 1));
+      if (cand && max_score < h) {
+        max_score = h;
+        max_i = i;
+        max_j = j;
+      }
+    }
+  }
+  if (max_i == 0 && max_j == 0) return res;  // :354-356: nothing, score untouched
+  res.has_score = true;
+  res.score = max_score;
+
+  // traceback (:361-437 linear, :547-676 affine, :777-923 convex)
+  std::vector<std::int32_t>& al = res.pairs;
+  std::int64_t i = max_i, j = max_j, prev_i = 0, prev_j = 0;
+  auto go_on = [&]() {
+    if (T == kSW) return at(H, i, j) != 0;
+    if (T == kNW) return !(i == 0 && j == 0);
+    return !(i == 0 || j == 0);
+  };
+  // guards a walk that finds no move (stale prev_*, as the reference keeps
+  // them) from looping forever on an inconsistent matrix
+  std::size_t steps = 0;
+  const std::size_t max_steps = static_cast<std::size_t>(R + W) * 4;
+  while (go_on()) {
+    if (++steps > max_steps) throw std::runtime_error("oracle: inconsistent SISD traceback");
+    const std::int32_t h = at(H, i, j);
+    bool found = false, ext_left = false, ext_up = false;
+    if (i != 0 && j != 0) {
+      for (auto p : preds[i]) {
+        if (h == at(H, p, j - 1) + prof(i, j)) {
+          prev_i = p;
+          prev_j = j - 1;
+          found = true;
+          break;
+        }
+      }
+    }
+    if (!found && i != 0) {
+      for (auto p : preds[i]) {
+        bool hit;
+        if (S == kLinear) {
+          hit = h == at(H, p, j) + g;
+        } else if (S == kAffine) {
+          hit = (ext_up = h == at(F, p, j) + e) || h == at(H, p, j) + g;
+        } else {
+          hit = (ext_up = ext_up || h == at(F, p, j) + e) || h == at(H, p, j) + g ||
+                (ext_up = ext_up || h == at(O, p, j) + c) || h == at(H, p, j) + q;
+        }
+        if (hit) {
+          prev_i = p;
+          prev_j = j;
+          found = true;
+          break;
+        }
+      }
+    }
+    if (!found && (S == kLinear || j != 0)) {
+      // linear tests the left cell without the column guard (:416); on a
+      // consistent matrix the up move always wins in column 0
+      const std::int32_t hl = H[i * W + j - 1];
+      bool hit;
+      if (S == kLinear) {
+        hit = h == hl + g;
+      } else if (S == kAffine) {
+        hit = (ext_left = h == at(E, i, j - 1) + e) || h == hl + g;
+      } else {
+        hit = (ext_left = h == at(E, i, j - 1) + e) || h == hl + g ||
+              (ext_left = ext_left || h == at(Q, i, j - 1) + c) || h == hl + q;
+      }
+      if (hit) {
+        prev_i = i;
+        prev_j = j - 1;
+        found = true;
+      }
+    }
+    al.push_back(i == prev_i ? -1 : node_id(i));
+    al.push_back(j == prev_j ? -1 : static_cast<std::int32_t>(j - 1));
+    i = prev_i;
+    j = prev_j;
+    if (ext_left) {
+      while (true) {
+        al.push_back(-1);
+        al.push_back(static_cast<std::int32_t>(j - 1));
+        --j;
+        const bool e_breaks = at(E, i, j) + e != at(E, i, j + 1);
+        const bool q_breaks = S != kConvex || at(Q, i, j) + c != at(Q, i, j + 1);
+        if (e_breaks && q_breaks) break;
+      }
+    } else if (ext_up) {
+      while (true) {
+        bool stop;
+        prev_i = 0;
+        if (S == kAffine) {
+          stop = false;
+          for (auto p : in_rows[i]) {
+            if ((stop = at(F, i, j) == at(H, p, j) + g) || at(F, i, j) == at(F, p, j) + e) {
+              prev_i = p;
+              break;
+            }
+          }
+        } else {
+          stop = true;
+          for (auto p : in_rows[i]) {
+            if (at(F, i, j) == at(F, p, j) + e || at(O, i, j) == at(O, p, j) + c) {
+              prev_i = p;
+              stop = false;
+              break;
+            }
+          }
+          if (stop) {
+            for (auto p : in_rows[i]) {
+              if (at(F, i, j) == at(H, p, j) + g || at(O, i, j) == at(H, p, j) + q) {
+                prev_i = p;
+                break;
+              }
+            }
+          }
+        }
+        al.push_back(node_id(i));
+        al.push_back(-1);
+        i = prev_i;
+        if (stop || i == 0) break;
+      }
+    }
+  }
+  // reverse pair order
+  std::vector<std::int32_t> rev(al.size());
+  const std::size_t np = al.size() / 2;
+  for (std::size_t k = 0; k < np; ++k) {
+    rev[2 * k] = al[2 * (np - 1 - k)];
+    rev[2 * k + 1] = al[2 * (np - 1 - k) + 1];
+  }
+  al.swap(rev);
+  return res;
+}
+
 // The strand-ambiguous caller's reverse complement (src/main.cpp:286, 296
 // call biosoup 0.10.0 Sequence::ReverseAndComplement, which the reference
 // does not vendor).  One rule shared by the product, this oracle, the
@@ -724,6 +1010,19 @@ void* oracle_engine_new(int type, int m, int n, int g, int e, int q, int c) {
   }
 }
 
+// The scalar engine: the same validation and normalisation as Create, then
+// SisdAlignmentEngine semantics for every Align (src/alignment_engine.cpp:68-71).
+void* oracle_sisd_engine_new(int type, int m, int n, int g, int e, int q, int c) {
+  try {
+    Engine* en = oracle::CreateEngine(type, m, n, g, e, q, c);
+    en->sisd = true;
+    return en;
+  } catch (std::exception& ex) {
+    oracle::g_error = ex.what();
+    return nullptr;
+  }
+}
+
 void oracle_engine_free(void* p) { delete static_cast<Engine*>(p); }
 
 int oracle_engine_subtype(void* p) { return static_cast<Engine*>(p)->subtype; }
@@ -756,7 +1055,9 @@ int oracle_align(void* eng, void* graph, const char* seq, std::uint32_t len,
                  std::int32_t* pairs_out, std::uint32_t cap, std::int32_t* score,
                  int* has_score, int* width_bits) {
   try {
-    auto r = oracle::Align(*static_cast<Engine*>(eng), *static_cast<Graph*>(graph), seq, len);
+    const Engine& en = *static_cast<Engine*>(eng);
+    auto r = en.sisd ? oracle::SisdAlign(en, *static_cast<Graph*>(graph), seq, len)
+                     : oracle::Align(en, *static_cast<Graph*>(graph), seq, len);
     std::uint32_t np = static_cast<std::uint32_t>(r.pairs.size() / 2);
     if (np > cap) return -2;
     std::memcpy(pairs_out, r.pairs.data(), r.pairs.size() * sizeof(std::int32_t));

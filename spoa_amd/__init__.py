@@ -53,7 +53,7 @@ class HipStats(ctypes.Structure):
 class BatchOpts(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int32) for k in
                 ("type", "m", "n", "g", "e", "q", "c", "strand", "msa", "threads", "device", "overlap",
-                 "graph", "resident")]
+                 "graph", "resident", "sisd")]
 
 
 class BatchStats(ctypes.Structure):
@@ -108,6 +108,8 @@ def lib():
     L.spoa_last_error.restype = ctypes.c_char_p
     L.spoa_engine_new.restype = ctypes.c_void_p
     L.spoa_engine_new.argtypes = [ctypes.c_int] * 7
+    L.spoa_sisd_engine_new.restype = ctypes.c_void_p
+    L.spoa_sisd_engine_new.argtypes = [ctypes.c_int] * 7
     L.spoa_engine_free.argtypes = [ctypes.c_void_p]
     L.spoa_engine_subtype.argtypes = [ctypes.c_void_p]
     L.spoa_engine_prealloc.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
@@ -288,12 +290,14 @@ class AlignmentEngine:
         self.last_width = 0
 
     @staticmethod
-    def create(type, m, n, g, e=None, q=None, c=None):
+    def create(type, m, n, g, e=None, q=None, c=None, sisd=False):
+        """sisd=True: the reference's scalar engine's results
+        (SisdAlignmentEngine, what a reference built without SIMD runs)."""
         if e is None:
             e = g
         if q is None:
             q, c = g, e
-        h = lib().spoa_engine_new(type, m, n, g, e, q, c)
+        h = (lib().spoa_sisd_engine_new if sisd else lib().spoa_engine_new)(type, m, n, g, e, q, c)
         if not h:
             raise ValueError(_error())
         return AlignmentEngine(h)
@@ -479,14 +483,15 @@ class WindowResults:
             self._h = None
 
 
-def _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap=False, graph=False, resident=False):
+def _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap=False, graph=False, resident=False,
+          sisd=False):
     if e is None:
         e = g
     if q is None:
         q, c = g, e
     return BatchOpts(type=type, m=m, n=n, g=g, e=e, q=q, c=c, strand=int(strand), msa=int(msa),
                      threads=threads, device=device, overlap=int(overlap), graph=int(graph),
-                     resident=int(resident))
+                     resident=int(resident), sisd=int(sisd))
 
 
 class BatchContext:
@@ -494,8 +499,8 @@ class BatchContext:
     between runs) for one set of scoring options."""
 
     def __init__(self, type, m, n, g, e=None, q=None, c=None, strand=False, msa=True, threads=0, device=0,
-                 overlap=False, graph=False, resident=False):
-        self.opts = _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap, graph, resident)
+                 overlap=False, graph=False, resident=False, sisd=False):
+        self.opts = _opts(type, m, n, g, e, q, c, strand, msa, threads, device, overlap, graph, resident, sisd)
         self._h = lib().spoa_batch_context_new(ctypes.byref(self.opts))
         if not self._h:
             raise ValueError(_error())
@@ -540,15 +545,16 @@ class BatchContext:
 
 
 def run_windows(windows, type, m, n, g, e=None, q=None, c=None, mode="batched", strand=False,
-                msa=True, threads=0, device=0):
+                msa=True, threads=0, device=0, sisd=False):
     """Aligns every read of every window in order into the window's graph,
     then builds its consensus (and MSA).  mode 'batched': one launch per
     round over all windows, graphs merged on the host; 'resident': the same
     launches with every window's graph kept, merged, sorted and summarised
-    on the device; 'per_call': the drop-in Align, one read at a time."""
+    on the device; 'per_call': the drop-in Align, one read at a time.
+    sisd=True: the reference's scalar engine's alignments."""
     if not isinstance(windows, Windows):
         windows = Windows.from_lists(windows)
-    o = _opts(type, m, n, g, e, q, c, strand, msa, threads, device, resident=mode == "resident")
+    o = _opts(type, m, n, g, e, q, c, strand, msa, threads, device, resident=mode == "resident", sisd=sisd)
     md = 1 if mode == "per_call" else 0
     h = lib().spoa_batch_run(md, ctypes.byref(o), windows.n_windows,
                              windows.win_read_off.ctypes.data, windows.read_off.ctypes.data,

@@ -41,8 +41,18 @@ std::unique_ptr<AlignmentEngine> AlignmentEngine::Create(AlignmentType type, std
 std::unique_ptr<AlignmentEngine> AlignmentEngine::Create(AlignmentType type, std::int8_t m, std::int8_t n,
                                                          std::int8_t g, std::int8_t e, std::int8_t q,
                                                          std::int8_t c) {
+  return CreateEngine(type, m, n, g, e, q, c, DefaultSisd());
+}
+
+std::unique_ptr<AlignmentEngine> CreateEngine(AlignmentType type, std::int8_t m, std::int8_t n, std::int8_t g,
+                                              std::int8_t e, std::int8_t q, std::int8_t c, bool sisd) {
   const AlignmentSubtype subtype = CheckCreateParams(type, &g, &e, &q, &c);
-  return std::unique_ptr<AlignmentEngine>(new HipAlignmentEngine(type, subtype, m, n, g, e, q, c));
+  return std::unique_ptr<AlignmentEngine>(new HipAlignmentEngine(type, subtype, m, n, g, e, q, c, sisd));
+}
+
+bool DefaultSisd() {
+  const char* env = std::getenv("SPOA_HIP_ENGINE");
+  return env && std::string(env) == "sisd";
 }
 
 AlignmentSubtype CheckCreateParams(AlignmentType type, std::int8_t* g_, std::int8_t* e_, std::int8_t* q_,
@@ -100,13 +110,14 @@ int DefaultDevice() {
 }
 
 HipAlignmentEngine::HipAlignmentEngine(AlignmentType type, AlignmentSubtype subtype, std::int8_t m, std::int8_t n,
-                                       std::int8_t g, std::int8_t e, std::int8_t q, std::int8_t c)
-    : AlignmentEngine(type, subtype, m, n, g, e, q, c) {
+                                       std::int8_t g, std::int8_t e, std::int8_t q, std::int8_t c, bool sisd)
+    : AlignmentEngine(type, subtype, m, n, g, e, q, c), sisd_(sisd) {
   spoa_hip_params p{};
   p.type = static_cast<std::int32_t>(type);
   p.subtype = static_cast<std::int32_t>(subtype);
   p.m = m; p.n = n; p.g = g; p.e = e; p.q = q; p.c = c;
-  const int rc = spoa_hip_engine_create(DefaultDevice(), &p, &device_);
+  const int rc = sisd ? spoa_hip_engine_create_sisd(DefaultDevice(), &p, &device_)
+                      : spoa_hip_engine_create(DefaultDevice(), &p, &device_);
   if (rc != SPOA_HIP_OK) Rethrow(rc);
 }
 
@@ -130,22 +141,24 @@ spoa_hip_graph HipAlignmentEngine::View(const Graph& graph) {
   return gv;
 }
 
-// :703-780
+// :703-780 (the scalar engine: sisd...:255-291)
 Alignment HipAlignmentEngine::Align(const char* sequence, std::uint32_t sequence_len, const Graph& graph,
                                     std::int32_t* score) {
+  const std::string who = sisd_ ? "[spoa::SisdAlignmentEngine::Align]" : "[spoa::SimdAlignmentEngine::Align]";
   if (sequence_len > static_cast<std::uint32_t>(std::numeric_limits<std::int32_t>::max())) {
-    throw std::invalid_argument("[spoa::SimdAlignmentEngine::Align] error: too large sequence!");
+    throw std::invalid_argument(who + " error: too large sequence!");
   }
   last_width_ = 0;
   if (graph.nodes().empty() || sequence_len == 0) return Alignment();
   {
-    const std::int64_t w = WorstCaseAlignmentScore(static_cast<std::int64_t>(sequence_len) + 8,
+    // the scalar engine bounds (n, V) and always scores in int32
+    const std::int64_t w = WorstCaseAlignmentScore(static_cast<std::int64_t>(sequence_len) + (sisd_ ? 0 : 8),
                                                    static_cast<std::int64_t>(graph.nodes().size()));
     last_width_ = w < std::numeric_limits<std::int32_t>::min() + 1024
                       ? 0
-                      : (w < std::numeric_limits<std::int16_t>::min() + 1024 ? 32 : 16);
+                      : (sisd_ || w < std::numeric_limits<std::int16_t>::min() + 1024 ? 32 : 16);
     // :723-725, before anything is sized for the call
-    if (last_width_ == 0) throw std::invalid_argument("[spoa::SimdAlignmentEngine::Align] error: possible overflow!");
+    if (last_width_ == 0) throw std::invalid_argument(who + " error: possible overflow!");
   }
   const spoa_hip_graph gv = View(graph);
   const std::size_t cap = static_cast<std::size_t>(gv.num_nodes) + sequence_len;

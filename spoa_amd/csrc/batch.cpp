@@ -65,7 +65,7 @@ std::string ReverseComplement(const char* s, std::size_t len) {
 // --------------------------------------------------------------- drivers --
 
 BatchContext::BatchContext(const BatchOptions& opt) : opt_(opt), pool_(opt.threads) {
-  for (auto& e : engines_) e = spoa::AlignmentEngine::Create(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c);
+  for (auto& e : engines_) e = spoa::CreateEngine(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c, opt.sisd);
   // the halves share the device memory: each engine gets half the budget;
   // their kernels take turns on one compute stream (engines_[0]'s, which is
   // destroyed last) while copies and host merging overlap
@@ -466,7 +466,7 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
 std::vector<WindowResult> RunPerCall(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats) {
   const auto t0 = Clock::now();
   BatchStats st;
-  auto engine = spoa::AlignmentEngine::Create(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c);
+  auto engine = spoa::CreateEngine(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c, opt.sisd);
   auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(engine.get());
   std::vector<WindowResult> out(ws.n_windows);
   for (std::uint32_t w = 0; w < ws.n_windows; ++w) {

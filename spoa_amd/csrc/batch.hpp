@@ -43,6 +43,7 @@ struct BatchOptions {
   std::uint32_t max_rounds = 0;  // stop after this many rounds (0: all; the autotuner's short runs)
   bool graph = false;      // launch every pass as a captured HIP graph
   bool resident = false;   // window graphs on the device: merges, consensus and MSA there too
+  bool sisd = false;       // the reference's scalar engine's alignments (SisdAlignmentEngine)
   unsigned threads = 0;    // host threads (0: all hardware threads)
   int device = 0;
 };

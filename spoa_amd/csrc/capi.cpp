@@ -61,6 +61,7 @@ spoa_amd::BatchOptions Options(const spoa_batch_opts* o) {
   b.overlap = o->overlap != 0;
   b.graph = o->graph != 0;
   b.resident = o->resident != 0;
+  b.sisd = o->sisd != 0;
   return b;
 }
 
@@ -78,6 +79,18 @@ void* spoa_engine_new(int type, int m, int n, int g, int e, int q, int c) {
                                          static_cast<std::int8_t>(n), static_cast<std::int8_t>(g),
                                          static_cast<std::int8_t>(e), static_cast<std::int8_t>(q),
                                          static_cast<std::int8_t>(c))
+        .release();
+  } catch (const std::exception& ex) {
+    Error(ex);
+    return nullptr;
+  }
+}
+
+void* spoa_sisd_engine_new(int type, int m, int n, int g, int e, int q, int c) {
+  try {
+    return spoa::CreateEngine(static_cast<spoa::AlignmentType>(type), static_cast<std::int8_t>(m),
+                              static_cast<std::int8_t>(n), static_cast<std::int8_t>(g), static_cast<std::int8_t>(e),
+                              static_cast<std::int8_t>(q), static_cast<std::int8_t>(c), true)
         .release();
   } catch (const std::exception& ex) {
     Error(ex);

@@ -40,11 +40,28 @@ inline bool HipOk(hipError_t rc, const char* what) {
   return false;
 }
 
-inline constexpr char kMsgTooLarge[] = "[spoa::SimdAlignmentEngine::Align] error: too large sequence!";
-inline constexpr char kMsgOverflow[] = "[spoa::SimdAlignmentEngine::Align] error: possible overflow!";
-inline constexpr char kMsgNoMemory[] = "[spoa::SimdAlignmentEngine::Align] error: insufficient memory!";
-inline constexpr char kMsgPreTooLarge[] = "[spoa::SimdAlignmentEngine::Prealloc] error: too large sequence!";
-inline constexpr char kMsgPreNoMemory[] = "[spoa::SimdAlignmentEngine::Prealloc] error: insufficient memory!";
+// The status strings of the engine an engine stands in for: the SIMD
+// engine's (simd...:381-422, :703-780) or the scalar engine's
+// (sisd_alignment_engine.cpp:62-77, :255-283).
+struct EngineMsgs {
+  const char* too_large;
+  const char* overflow;
+  const char* no_memory;
+  const char* pre_too_large;
+  const char* pre_no_memory;
+};
+inline constexpr EngineMsgs kSimdMsgs{
+    "[spoa::SimdAlignmentEngine::Align] error: too large sequence!",
+    "[spoa::SimdAlignmentEngine::Align] error: possible overflow!",
+    "[spoa::SimdAlignmentEngine::Align] error: insufficient memory!",
+    "[spoa::SimdAlignmentEngine::Prealloc] error: too large sequence!",
+    "[spoa::SimdAlignmentEngine::Prealloc] error: insufficient memory!"};
+inline constexpr EngineMsgs kSisdMsgs{
+    "[spoa::SisdAlignmentEngine::Align] error: too large sequence!",
+    "[spoa::SisdAlignmentEngine::Align] error: possible overflow!",
+    "[spoa::SisdAlignmentEngine::Align] error: insufficient memory!",
+    "[spoa::SisdAlignmentEngine::Prealloc] error: too large sequence!",
+    "[spoa::SisdAlignmentEngine::Prealloc] error: insufficient memory!"};
 
 constexpr uint64_t kAlign = 256;
 inline uint64_t RoundUp(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
@@ -78,18 +95,35 @@ inline int Lanes(int width_bits) { return width_bits == 16 ? 16 : 8; }
 // (kept at 30000 below int16's maximum).
 struct Shape {
   int width = 0, bits = 0, lanes = 0;
+  int cfgw = 0;  // the width whose launch configuration the job takes
 };
 
-inline Shape JobShape(const spoa_hip_params& p, uint32_t len, uint32_t V) {
+// With `sisd` the job reproduces the reference's scalar engine
+// (sisd_alignment_engine.cpp): its overflow bound is on (n, V) and it
+// scores in int32 (the width reported); its rows have no padding and its
+// convex E / Q open from the final H of every column, i.e. one-column
+// segments (lanes = 1).  Its scores are those of the SIMD engine's recurrences,
+// so they fit int16 rows wherever the SIMD engine's int16 bound holds (or
+// the local-alignment rule above), and such jobs take the int16 launch
+// configuration.
+inline Shape JobShape(const spoa_hip_params& p, uint32_t len, uint32_t V, bool sisd) {
   Shape sh;
+  const bool sw16 = p.type == SPOA_HIP_SW &&
+                    static_cast<int64_t>(std::max<int32_t>(p.m, 0)) * (static_cast<int64_t>(len) + 8) <= 30000;
+  if (sisd) {
+    if (WorstCase(p, len, V) < std::numeric_limits<int32_t>::min() + 1024) return sh;
+    sh.width = 32;
+    sh.lanes = 1;
+    sh.bits = WidthBits(p, len, V) == 16 || sw16 ? 16 : 32;
+    sh.cfgw = sh.bits;
+    return sh;
+  }
   sh.width = WidthBits(p, len, V);
   if (sh.width == 0) return sh;
   sh.lanes = Lanes(sh.width);
   sh.bits = sh.width;
-  if (sh.width == 32 && p.type == SPOA_HIP_SW &&
-      static_cast<int64_t>(std::max<int32_t>(p.m, 0)) * (static_cast<int64_t>(len) + 8) <= 30000) {
-    sh.bits = 16;
-  }
+  sh.cfgw = sh.width;
+  if (sh.width == 32 && sw16) sh.bits = 16;
   return sh;
 }
 inline int Matrices(int subtype) { return subtype == 0 ? 1 : (subtype == 1 ? 3 : 5); }
@@ -173,8 +207,9 @@ inline void MpMul(const int32_t* a, const int32_t* b, int32_t* out) {
   std::memcpy(out, r, sizeof(r));
 }
 
-inline KParams MakeParams(const spoa_hip_params& p, int width_bits, int lanes, int k) {
+inline KParams MakeParams(const spoa_hip_params& p, int width_bits, int lanes, int k, bool sisd) {
   KParams kp{};
+  kp.sisd = sisd ? 1 : 0;
   kp.type = p.type;
   kp.m = p.m;
   kp.mis = p.n;
@@ -260,6 +295,10 @@ struct spoa_hip_engine {
   bool graph_pass = false;  // the pending pass ran as a graph
   std::vector<int> pass_width;  // score width of every launch of the pass
   spoa_hip_params prm{};
+  // the reference's scalar engine's results and messages instead of its
+  // SIMD engine's (spoa_hip_engine_create_sisd)
+  bool sisd = false;
+  const spoa_hip_host::EngineMsgs* msg = &spoa_hip_host::kSimdMsgs;
   int num_cus = 256;
   uint64_t budget = 0;
   DevBuf d_in, d_out, d_counter;

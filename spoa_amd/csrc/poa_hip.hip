@@ -26,7 +26,7 @@
 
 #include "engine.hpp"
 #include "pass_split.hpp"
-#include "poa_kernels.hpp"
+#include "pick_kernel.hpp"
 #include "spoa_hip.h"
 #include "thread_pool.hpp"
 
@@ -34,43 +34,38 @@ using namespace spoa_hip_host;
 
 namespace spoa_hip_host {
 
-template <int SUB, typename ST, int K, int L>
-KernelFn PickMaxT(int waves) {
-  if (waves == 1) return spoa_hip_dev::PoaAlignKernel<SUB, ST, K, 64, L>;
-  if constexpr (K == 32) {
-    return nullptr;  // 32 columns a lane only in single-wave workgroups
-  } else {
-    return spoa_hip_dev::PoaAlignKernel<SUB, ST, K, 64 * kMaxGroupWaves, L>;
-  }
-}
-
-template <int SUB, typename ST, int L>
-KernelFn PickK(int k, int waves) {
-  if (k == 32) return PickMaxT<SUB, ST, 32, L>(waves);
-  if (k == 16) return PickMaxT<SUB, ST, 16, L>(waves);
-  if constexpr (SUB == 2 && L == 16) {
-    return nullptr;  // convex lanes hold whole 16-column segments
-  } else {
-    return PickMaxT<SUB, ST, 8, L>(waves);
-  }
-}
-
 // bits: stored rows; lanes: the reference lane count the launch reproduces.
 // Only convex rows depend on the lane count beyond the row width each job
 // carries, so linear and affine rows always run the kernel of their width's
 // own lane count; convex int16 rows with 8 lanes hold the local alignments
-// the reference scores at int32.
+// the reference scores at int32, and one-column segments are the scalar
+// engine's convex rows (their kernels are built in poa_sisd.hip).
 KernelFn PickKernel(int subtype, int bits, int lanes, int k, int waves) {
   switch (subtype) {
     case 0: return bits == 16 ? PickK<0, int16_t, 16>(k, waves) : PickK<0, int32_t, 8>(k, waves);
     case 1: return bits == 16 ? PickK<1, int16_t, 16>(k, waves) : PickK<1, int32_t, 8>(k, waves);
     default:
+      if (lanes == 1) return PickConvexFullGotoh(bits, k, waves);
       if (bits == 32) return PickK<2, int32_t, 8>(k, waves);
       if (lanes == 8) return PickK<2, int16_t, 8>(k, waves);
       return PickK<2, int16_t, 16>(k, waves);
   }
 }
 
+
+}  // namespace spoa_hip_host
+
+namespace spoa_hip_dev {
+
+// Zeroes the job counters of a pass (a kernel rather than a memset so the
+// launch sequence can be captured in a graph).
+__global__ void ResetCounters(uint32_t* p, uint32_t n) {
+  if (blockIdx.x == 0 && threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+}  // namespace spoa_hip_dev
+
+namespace spoa_hip_host {
 
 bool LaunchResetCounters(uint32_t* counters, uint32_t n, hipStream_t stream) {
   spoa_hip_dev::ResetCounters<<<1, 256, 0, stream>>>(counters, n);
@@ -111,7 +106,7 @@ int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::ve
       // fragmented: start over with one segment of twice what the whole
       // set needs (within the budget), so that sets which grow round by
       // round do not free and allocate the arena every round
-      if (retried) return Fail(SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+      if (retried) return Fail(SPOA_HIP_NO_MEMORY, E->msg->no_memory);
       retried = true;
       for (auto& sg : E->arena) sg.Release();
       E->arena.clear();
@@ -138,7 +133,7 @@ int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::ve
       want = rest;
       if (want < need[x] || hipMalloc(&sg.p, want) != hipSuccess) {
         (void)hipGetLastError();
-        return Fail(SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+        return Fail(SPOA_HIP_NO_MEMORY, E->msg->no_memory);
       }
     }
     sg.size = want;
@@ -203,7 +198,7 @@ int BuildGroups(spoa_hip_engine* E, const std::vector<Plan>& plans, size_t lo, s
     }
     gp.grid = std::max<uint32_t>(
         1u, std::min<uint32_t>(n, static_cast<uint32_t>(std::max(1, per_cu)) * static_cast<uint32_t>(E->num_cus)));
-    gp.prm = MakeParams(P, plans[x].bits, plans[x].lanes, plans[x].cfg.k);
+    gp.prm = MakeParams(P, plans[x].bits, plans[x].lanes, plans[x].cfg.k, E->sisd);
     // the group's jobs of at least half its largest job's V * n (they lead
     // the queue) get the raised wave priority (1.7% less kernel time on
     // 2,048 local convex 1.5 kb windows; raising the side launches' jobs
@@ -557,8 +552,8 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
 
   const auto t_alloc0 = std::chrono::steady_clock::now();
   if (!E->h_in.Reserve(in_bytes) || !E->h_out.Reserve(out_bytes)) return SPOA_HIP_DEVICE_ERROR;
-  if (!E->d_in.Reserve(in_bytes, kMsgNoMemory) || !E->d_out.Reserve(out_bytes, kMsgNoMemory) ||
-      !E->d_counter.Reserve(256 * sizeof(uint32_t), kMsgNoMemory)) {
+  if (!E->d_in.Reserve(in_bytes, E->msg->no_memory) || !E->d_out.Reserve(out_bytes, E->msg->no_memory) ||
+      !E->d_counter.Reserve(256 * sizeof(uint32_t), E->msg->no_memory)) {
     return SPOA_HIP_NO_MEMORY;
   }
   // place every job's matrices in an arena segment (first fit in order)
@@ -897,8 +892,8 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
       if (j.graph.num_nodes == 0 || j.len == 0 || j.len > static_cast<uint32_t>(std::numeric_limits<int32_t>::max())) {
         continue;
       }
-      const Shape sh = JobShape(P, j.len, j.graph.num_nodes);
-      if (sh.width != 0) cfg_len[sh.width == 32] = std::max(cfg_len[sh.width == 32], j.len);
+      const Shape sh = JobShape(P, j.len, j.graph.num_nodes, E->sisd);
+      if (sh.width != 0) cfg_len[sh.cfgw == 32] = std::max(cfg_len[sh.cfgw == 32], j.len);
     }
   }
   std::vector<Plan> plans;
@@ -911,14 +906,14 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     j.status = SPOA_HIP_OK;
     // reference simd...:708-715
     if (j.len > static_cast<uint32_t>(std::numeric_limits<int32_t>::max())) {
-      fail(j, SPOA_HIP_TOO_LARGE, kMsgTooLarge);
+      fail(j, SPOA_HIP_TOO_LARGE, E->msg->too_large);
       continue;
     }
     const uint32_t V = j.graph.num_nodes;
     if (V == 0 || j.len == 0) continue;
-    const Shape sh = JobShape(P, j.len, V);
+    const Shape sh = JobShape(P, j.len, V, E->sisd);
     if (sh.width == 0) {
-      fail(j, SPOA_HIP_OVERFLOW, kMsgOverflow);
+      fail(j, SPOA_HIP_OVERFLOW, E->msg->overflow);
       continue;
     }
     Plan pl;
@@ -929,7 +924,7 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     // launch configuration of the reference width (int16 rows of the
     // reference's int32 jobs take the int32 launch shape: they form the
     // same few, long launches)
-    pl.cfg = ChooseCfg(E, batch ? cfg_len[sh.width == 32] : j.len, sh.width, batch);
+    pl.cfg = ChooseCfg(E, batch ? cfg_len[sh.cfgw == 32] : j.len, sh.cfgw, batch);
     pl.work = static_cast<uint64_t>(V) * j.len;
     const int L = sh.lanes;
     pl.W = (j.len + L - 1) / L * L;
@@ -939,7 +934,7 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     pl.graph_words = RoundUp(9ull * V + 4 + E_in, 4);
     pl.seq_bytes = RoundUp(pl.stride + 16, 16);
     if (!spoa_amd::PassItemFits(PassItemOf(pl, j), E->budget)) {
-      fail(j, SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+      fail(j, SPOA_HIP_NO_MEMORY, E->msg->no_memory);
       continue;
     }
     plans.push_back(pl);
@@ -1028,6 +1023,14 @@ int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_e
   return SPOA_HIP_OK;
 }
 
+int spoa_hip_engine_create_sisd(int device, const spoa_hip_params* params, spoa_hip_engine** out) {
+  const int rc = spoa_hip_engine_create(device, params, out);
+  if (rc != SPOA_HIP_OK) return rc;
+  (*out)->sisd = true;
+  (*out)->msg = &kSisdMsgs;
+  return SPOA_HIP_OK;
+}
+
 int spoa_hip_engine_destroy(spoa_hip_engine* E) {
   if (!E) return SPOA_HIP_OK;
   (void)hipSetDevice(E->device);
@@ -1065,24 +1068,28 @@ int spoa_hip_engine_destroy(spoa_hip_engine* E) {
   return SPOA_HIP_OK;
 }
 
-// reference simd...:381-422
+// reference simd...:381-422; the scalar engine (sisd...:62-77) checks only
+// the length and sizes int32 rows with no overflow bound
 int spoa_hip_prealloc(spoa_hip_engine* E, uint32_t max_len, uint32_t alphabet_size) {
   if (!E) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null engine");
   if (max_len > static_cast<uint32_t>(std::numeric_limits<int32_t>::max())) {
-    return Fail(SPOA_HIP_TOO_LARGE, kMsgPreTooLarge);
+    return Fail(SPOA_HIP_TOO_LARGE, E->msg->pre_too_large);
   }
   const int64_t V = static_cast<int64_t>(max_len) * (alphabet_size & 0xffu);
-  const int64_t w = WorstCase(E->prm, static_cast<int64_t>(max_len) + 8, V);
-  if (w < std::numeric_limits<int32_t>::min() + 1024) return SPOA_HIP_OK;
-  const int width = w < std::numeric_limits<int16_t>::min() + 1024 ? 32 : 16;
+  int width = 32;
+  if (!E->sisd) {
+    const int64_t w = WorstCase(E->prm, static_cast<int64_t>(max_len) + 8, V);
+    if (w < std::numeric_limits<int32_t>::min() + 1024) return SPOA_HIP_OK;
+    width = w < std::numeric_limits<int16_t>::min() + 1024 ? 32 : 16;
+  }
   if (!HipOk(hipSetDevice(E->device), "hipSetDevice")) return SPOA_HIP_DEVICE_ERROR;
   const Cfg cfg = ChooseCfg(E, max_len, width, false);
-  const int L = Lanes(width);  // the reference's sizing (int32 rows at most)
+  const int L = E->sisd ? 1 : Lanes(width);  // the reference's sizing (int32 rows at most)
   const uint64_t W = (static_cast<uint64_t>(max_len) + L - 1) / L * L;
   const uint64_t stride = RoundUp(W, static_cast<uint64_t>(cfg.k));
-  if (V >= static_cast<int64_t>(std::numeric_limits<uint32_t>::max())) return Fail(SPOA_HIP_NO_MEMORY, kMsgPreNoMemory);
+  if (V >= static_cast<int64_t>(std::numeric_limits<uint32_t>::max())) return Fail(SPOA_HIP_NO_MEMORY, E->msg->pre_no_memory);
   const uint64_t bytes = JobBytes(E->prm, static_cast<uint32_t>(V), static_cast<uint32_t>(stride), width, cfg);
-  if (bytes > E->budget) return Fail(SPOA_HIP_NO_MEMORY, kMsgPreNoMemory);
+  if (bytes > E->budget) return Fail(SPOA_HIP_NO_MEMORY, E->msg->pre_no_memory);
   uint64_t largest = 0;
   for (const auto& sg : E->arena) largest = std::max(largest, sg.size);
   if (largest < bytes) {
@@ -1099,7 +1106,7 @@ int spoa_hip_prealloc(spoa_hip_engine* E, uint32_t max_len, uint32_t alphabet_si
     DevBuf sg;
     if (hipMalloc(&sg.p, bytes) != hipSuccess) {
       (void)hipGetLastError();
-      return Fail(SPOA_HIP_NO_MEMORY, kMsgPreNoMemory);
+      return Fail(SPOA_HIP_NO_MEMORY, E->msg->pre_no_memory);
     }
     sg.size = bytes;
     E->arena.push_back(sg);
@@ -1135,16 +1142,16 @@ int spoa_hip_align_noalloc(spoa_hip_engine* E, const spoa_hip_graph* graph, cons
   if (!base) return Fail(SPOA_HIP_DEVICE_ERROR, "[spoa::hip] no scratch address for the device");
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : E->stream;
   // reference simd...:708-715
-  if (len > static_cast<uint32_t>(std::numeric_limits<int32_t>::max())) return Fail(SPOA_HIP_TOO_LARGE, kMsgTooLarge);
+  if (len > static_cast<uint32_t>(std::numeric_limits<int32_t>::max())) return Fail(SPOA_HIP_TOO_LARGE, E->msg->too_large);
   const uint32_t V = graph->num_nodes;
   if (V == 0 || len == 0) {
     spoa_hip_dev::NoAllocEmpty<<<1, 64, 0, st>>>(result);
     return HipOk(hipGetLastError(), "kernel launch") ? SPOA_HIP_OK : SPOA_HIP_DEVICE_ERROR;
   }
-  const Shape sh = JobShape(E->prm, len, V);
-  if (sh.width == 0) return Fail(SPOA_HIP_OVERFLOW, kMsgOverflow);
+  const Shape sh = JobShape(E->prm, len, V, E->sisd);
+  if (sh.width == 0) return Fail(SPOA_HIP_OVERFLOW, E->msg->overflow);
   // the per-call launch configuration of spoa_hip_align
-  const Cfg cfg = ChooseCfg(E, len, sh.width, false);
+  const Cfg cfg = ChooseCfg(E, len, sh.cfgw, false);
   const uint32_t W = (len + sh.lanes - 1) / sh.lanes * sh.lanes;
   const uint64_t stride = RoundUp(W, static_cast<uint64_t>(cfg.k));
   // scratch layout: counter | job record | result record | read | matrices | graph block
@@ -1188,7 +1195,7 @@ int spoa_hip_align_noalloc(spoa_hip_engine* E, const spoa_hip_graph* graph, cons
   spoa_hip_dev::NoAllocPack<<<1, 256, 0, st>>>(pa);
   if (!HipOk(hipGetLastError(), "kernel launch")) return SPOA_HIP_DEVICE_ERROR;
   KArgs args{};
-  args.prm = MakeParams(E->prm, sh.bits, sh.lanes, cfg.k);
+  args.prm = MakeParams(E->prm, sh.bits, sh.lanes, cfg.k, E->sisd);
   args.jobs = pa.job;
   args.n_jobs = 1;
   args.queue = pa.counter;
@@ -1329,9 +1336,9 @@ uint64_t spoa_hip_memory_in_use(spoa_hip_engine* E) { return E ? E->arena_bytes 
 
 uint64_t spoa_hip_job_footprint(spoa_hip_engine* E, uint32_t V, uint32_t len, int32_t batch) {
   if (!E || V == 0 || len == 0) return 0;
-  const Shape sh = JobShape(E->prm, len, V);
+  const Shape sh = JobShape(E->prm, len, V, E->sisd);
   if (sh.width == 0) return 0;
-  const Cfg cfg = ChooseCfg(E, len, sh.width, batch != 0);
+  const Cfg cfg = ChooseCfg(E, len, sh.cfgw, batch != 0);
   const int L = sh.lanes;
   const uint32_t W = (len + L - 1) / L * L;
   const uint32_t stride = static_cast<uint32_t>(RoundUp(W, static_cast<uint64_t>(cfg.k)));

@@ -167,6 +167,13 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
   constexpr int VEC = 16 / S;  // columns a 16-byte load
   const int32_t g = prm.g, e = prm.e, q = prm.q, c = prm.c;
   const int type = prm.type;
+  // The scalar engine's walk (sisd...:361-437, :547-676, :777-923): the
+  // same tests in the same order, but its global walk runs on through row 0
+  // and the first column (column -1 here) in the main loop, its convex left
+  // walk goes on while either gap chain holds, and E / Q of the first
+  // column are -inf (0 in row 0).
+  const bool sisd = prm.sisd != 0;
+  constexpr int32_t kNegInf = -2147483647 - 1 + 1024;
   auto U = [](int32_t v) -> int32_t { return static_cast<int32_t>(Uniform(static_cast<uint32_t>(v))); };
 
   // Tile geometry from the LDS the kernel lends (its query profile, dead
@@ -284,6 +291,22 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     return static_cast<int32_t>(
         mat[static_cast<uint64_t>(r) * rstride + static_cast<uint64_t>(t) * stride + static_cast<uint32_t>(col)]);
   };
+  // matrix t of row r at column col >= -1: column -1 is the first column
+  // (H, F, O from the graph block; E, Q -inf there, 0 in row 0)
+  auto cell = [&](int t, int32_t r, int32_t col) -> int32_t {
+    if (col >= 0) return at(t, r, col);
+    if (t == 0) return U(gv.fcH[r]);
+    if (t == 1) return U(gv.fcF[r]);
+    if (t == 3) return U(gv.fcO[r]);
+    return r == 0 ? 0 : kNegInf;
+  };
+  auto cell_v = [&](int t, int32_t r, int32_t col) -> int32_t {
+    if (col >= 0) return at_v(t, r, col);
+    if (t == 0) return gv.fcH[r];
+    if (t == 1) return gv.fcF[r];
+    if (t == 3) return gv.fcO[r];
+    return r == 0 ? 0 : kNegInf;
+  };
   auto pred_v = [&](uint32_t k) -> int32_t {
     return static_cast<int32_t>(k - T.pb0 < T.pn ? T.pr[k - T.pb0] : gv.preds[k]);
   };
@@ -321,9 +344,22 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
   int32_t i = static_cast<int32_t>(Uniform(static_cast<uint32_t>(max_i)));
   int32_t j = static_cast<int32_t>(Uniform(static_cast<uint32_t>(max_j)));
   int32_t prev_i = 0, prev_j = 0;
+  // the scalar engine's global walk ends only at the corner
+  const bool corner_walk = sisd && type == 1;
   while (true) {
     // every step emits one pair, so a full buffer also bounds the walk
-    if (j == -1 || i == 0 || status != kDevOk) break;
+    if (status != kDevOk) break;
+    if (corner_walk ? (i == 0 && j == -1) : (j == -1 || i == 0)) break;
+    if (corner_walk && i == 0) {
+      // Row 0 of the scalar walk: its cells hold the gap scores of the
+      // read's prefix, so every step (a gap opened or extended, one column
+      // each) moves left until column -1, as the SIMD engine's tail does.
+      while (j != -1 && status == kDevOk) {
+        emit(-1, j);
+        --j;
+      }
+      break;
+    }
     ensure(i, j);
     // Diagonal run through first predecessors, all in the tile (the
     // diagonal test below with k = 0 and j >= 1): each step reads the
@@ -357,15 +393,16 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     }
     // The general step: one lane a predecessor, 64 at a time; the first
     // hit in predecessor order wins, as in the serial loops.
-    const int32_t h = at(0, i, j);
+    const int32_t h = cell(0, i, j);
     if (type == 0 && h == 0) break;
-    const uint32_t pb = pred_begin(i), pe = pred_end(i);
+    // row 0 has no predecessors (the scalar walk along it moves left only)
+    const uint32_t pb = i > 0 ? pred_begin(i) : 0u, pe = i > 0 ? pred_end(i) : 0u;
     const bool source = pb == pe;
-    const uint32_t npr = source ? 1u : pe - pb;
+    const uint32_t npr = i == 0 ? 0u : (source ? 1u : pe - pb);
     bool found = false, ext_left = false, ext_up = false;
-    // diagonal (:1053-1063)
-    const int32_t pr = prof(i, j);
-    for (uint32_t k0 = 0; k0 < npr && !found; k0 += 64) {
+    // diagonal (:1053-1063); none from the first column
+    const int32_t pr = j >= 0 && i > 0 ? prof(i, j) : 0;
+    for (uint32_t k0 = 0; k0 < npr && !found && j >= 0; k0 += 64) {
       const uint32_t k = k0 + static_cast<uint32_t>(lane);
       const bool act = k < npr;
       const int32_t p = act && !source ? pred_v(pb + k) : 0;
@@ -386,15 +423,15 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
       const int32_t p = act && !source ? pred_v(pb + k) : 0;
       bool hit = false, ext = false;
       if (act) {
-        const int32_t hu = at_v(0, p, j);
+        const int32_t hu = cell_v(0, p, j);
         if (SUB == 0) {
           hit = h == hu + g;
         } else if (SUB == 1) {
-          const bool fh = h == at_v(1, p, j) + e;
+          const bool fh = h == cell_v(1, p, j) + e;
           hit = fh || h == hu + g;
           ext = fh;
         } else {
-          const bool fh = h == at_v(1, p, j) + e, gh = h == hu + g, oh = h == at_v(3, p, j) + c,
+          const bool fh = h == cell_v(1, p, j) + e, gh = h == hu + g, oh = h == cell_v(3, p, j) + c,
                      qh = h == hu + q;
           hit = fh || gh || oh || qh;
           ext = fh || (!gh && oh);
@@ -409,18 +446,20 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
         found = true;
       }
     }
-    // left (:1076-1083, :1459-1468, :1977-1990); column 0 reads the first column
-    if (!found) {
-      const int32_t hl = j > 0 ? at(0, i, j - 1) : gv.fcH[i];
+    // left (:1076-1083, :1459-1468, :1977-1990); column 0 reads the first
+    // column (the SIMD engine's E / Q there are its H, the scalar engine's
+    // -inf); none from the first column itself
+    if (!found && j >= 0) {
+      const int32_t hl = j > 0 ? at(0, i, j - 1) : U(gv.fcH[i]);
       bool hit;
       if (SUB == 0) {
         hit = h == hl + g;
       } else {
-        const int32_t el = j > 0 ? at(2, i, j - 1) : gv.fcH[i];
+        const int32_t el = j > 0 ? at(2, i, j - 1) : (sisd ? cell(2, i, -1) : U(gv.fcH[i]));
         if (SUB == 1) {
           hit = (ext_left = (h == el + e)) || h == hl + g;
         } else {
-          const int32_t ql = j > 0 ? at(4, i, j - 1) : gv.fcH[i];
+          const int32_t ql = j > 0 ? at(4, i, j - 1) : (sisd ? cell(4, i, -1) : U(gv.fcH[i]));
           hit = (ext_left = (h == el + e)) || h == hl + g || (ext_left = (h == ql + c)) || h == hl + q;
         }
       }
@@ -435,17 +474,23 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     j = prev_j;
 
     if (ext_left) {
-      // :1484-1509, :2006-2039 -- convex stops when either chain breaks.
-      // The walk emits columns j, j - 1, ... and stops at the first column
-      // jj < j (going left) that is -1 or where the chain breaks; the
-      // chain is tested 64 columns at a time.
+      // :1484-1509, :2006-2039 -- convex stops when either chain breaks
+      // (the scalar engine: when both break, sisd...:880-887).  The walk
+      // emits columns j, j - 1, ... and stops at the first column jj < j
+      // (going left) that is -1 or where the chain breaks; the chain is
+      // tested 64 columns at a time.
       int32_t jj = j - 1;
       while (true) {
         const int32_t col = jj - lane;
         bool brk = col < 0;
         if (!brk) {
-          brk = at_v(2, i, col) + e != at_v(2, i, col + 1) ||
-                (SUB == 2 && at_v(4, i, col) + c != at_v(4, i, col + 1));
+          const bool eb = at_v(2, i, col) + e != at_v(2, i, col + 1);
+          if (SUB == 2) {
+            const bool qb = at_v(4, i, col) + c != at_v(4, i, col + 1);
+            brk = sisd ? (eb && qb) : (eb || qb);
+          } else {
+            brk = eb;
+          }
         }
         const uint64_t bm = __ballot(brk);
         if (bm != 0) {
@@ -468,7 +513,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
         ensure(i, j);
         const uint32_t qb = pred_begin(i), qe = pred_end(i);
         const uint32_t nq = qe - qb;
-        const int32_t fv = at(1, i, j);
+        const int32_t fv = cell(1, i, j);
         bool stop;
         prev_i = 0;
         if (SUB == 1) {
@@ -476,8 +521,8 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
           for (uint32_t k0 = 0; k0 < nq; k0 += 64) {
             const uint32_t k = k0 + static_cast<uint32_t>(lane);
             const int32_t p = k < nq ? pred_v(qb + k) : 0;
-            const bool gh = k < nq && fv == at_v(0, p, j) + g;
-            const bool fh = k < nq && fv == at_v(1, p, j) + e;
+            const bool gh = k < nq && fv == cell_v(0, p, j) + g;
+            const bool fh = k < nq && fv == cell_v(1, p, j) + e;
             const uint64_t hm = __ballot(gh || fh);
             if (hm != 0) {
               const int w = static_cast<int>(__builtin_ctzll(hm));
@@ -487,12 +532,12 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
             }
           }
         } else {
-          const int32_t ov = at(3, i, j);
+          const int32_t ov = cell(3, i, j);
           stop = true;
           for (uint32_t k0 = 0; k0 < nq && stop; k0 += 64) {
             const uint32_t k = k0 + static_cast<uint32_t>(lane);
             const int32_t p = k < nq ? pred_v(qb + k) : 0;
-            const bool hit = k < nq && (fv == at_v(1, p, j) + e || ov == at_v(3, p, j) + c);
+            const bool hit = k < nq && (fv == cell_v(1, p, j) + e || ov == cell_v(3, p, j) + c);
             const uint64_t hm = __ballot(hit);
             if (hm != 0) {
               prev_i = __builtin_amdgcn_readlane(p, static_cast<int>(__builtin_ctzll(hm)));
@@ -505,7 +550,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
               const int32_t p = k < nq ? pred_v(qb + k) : 0;
               bool hit = false;
               if (k < nq) {
-                const int32_t hv = at_v(0, p, j);
+                const int32_t hv = cell_v(0, p, j);
                 hit = fv == hv + g || ov == hv + q;
               }
               const uint64_t hm = __ballot(hit);
@@ -523,8 +568,9 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     }
   }
 
-  // NW: finish along row 0 / column 0 (:1102-1124, :1552-1574, :2098-2121)
-  if (type == 1) {
+  // NW: finish along row 0 / column 0 (:1102-1124, :1552-1574, :2098-2121);
+  // the scalar engine's main loop walked them already
+  if (type == 1 && !sisd) {
     while (i == 0 && j != -1 && status == kDevOk) {
       emit(-1, j);
       --j;
@@ -770,6 +816,7 @@ PoaAlignKernel(const KArgs a) {
   const int wave = tid >> 6;
   const int nwaves = nthreads >> 6;
   const int type = prm.type;
+  const bool sisd = prm.sisd != 0;
   const int32_t m = prm.m, mis = prm.mis, g = prm.g, e = prm.e, q = prm.q, cc = prm.c;
   const int32_t pad = prm.pad;
   const uint32_t G2 = Splat<ST>(g), E2 = Splat<ST>(e), Q2 = Splat<ST>(q), C2 = Splat<ST>(cc);
@@ -896,7 +943,8 @@ PoaAlignKernel(const KArgs a) {
 
     // End-cell candidates per lane: the lane's best (floored) value and the
     // first row reaching it.  SW starts at 0, OV and NW at the type's -inf.
-    int32_t best = type == 0 ? 0 : prm.ninf;
+    // The scalar engine starts OV and NW at its int32 -inf (sisd...:13-14).
+    int32_t best = type == 0 ? 0 : (sisd ? kMinI + 1024 : prm.ninf);
     int32_t best_row = -1;
     int par = 0;
     int32_t fch_prev = gv.fcH[0];
@@ -1090,6 +1138,13 @@ PoaAlignKernel(const KArgs a) {
         for (int i = 0; i < KW; ++i) {
           h1[i] = WMax<ST>(h1[i], fv[i]);
           if (SUB == 2) h1[i] = WMax<ST>(h1[i], ov[i]);
+        }
+        // The scalar engine opens horizontal gaps from the clamped local H
+        // (sisd...:530-533, :752-767); clamping H1 first gives the same H
+        // and those gap scores.
+        if (sisd && type == 0) {
+#pragma unroll
+          for (int i = 0; i < KW; ++i) h1[i] = WMax<ST>(h1[i], 0u);
         }
         if (single) {
 #pragma unroll
@@ -1442,8 +1497,21 @@ PoaAlignKernel(const KArgs a) {
         }
 
         // end-cell candidates (:903-928): SW all rows, OV sink rows (row
-        // maxima floored at 0, :252-262), NW sink rows at column n - 1
-        if (active) {
+        // maxima floored at 0, :252-262), NW sink rows at column n - 1.
+        // The scalar engine takes the first strictly better cell in
+        // row-major order over the read's columns only, without the floor
+        // (sisd...:296-311): a row's maximum over columns < n.
+        if (active && sisd && (type == 0 || (type == 2 && sink))) {
+          int32_t lm = kMinI;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if (c0 + k < n) lm = max(lm, Col<ST, KW>(hw, k));
+          }
+          if (best < lm) {
+            best = lm;
+            best_row = static_cast<int32_t>(r);
+          }
+        } else if (active) {
           if (type == 0 || (type == 2 && sink)) {
             // whole words up to the reference width W (a multiple of 8)
             const int32_t wv = (static_cast<int32_t>(W) - static_cast<int32_t>(c0)) / CPW;
@@ -1550,13 +1618,15 @@ PoaAlignKernel(const KArgs a) {
     __syncthreads();
 
     // max_j (:937-953): NW the last column; SW/OV the first column of row
-    // max_i (padding included) holding the best value, -1 if none
+    // max_i (padding included; the scalar engine: the read's columns)
+    // holding the best value, -1 if none
     const uint64_t rstride = static_cast<uint64_t>(stride) * NM;
     int32_t max_j = static_cast<int32_t>(n) - 1;
     if (found && type != 1) {
       const ST* __restrict__ hrow = mat + static_cast<uint64_t>(max_i) * rstride;
+      const uint32_t cols = sisd ? n : W;
       int32_t first = 0x7fffffff;
-      for (uint32_t col = tid; col < W; col += nthreads) {
+      for (uint32_t col = tid; col < cols; col += nthreads) {
         if (static_cast<int32_t>(hrow[col]) == max_score) {
           first = static_cast<int32_t>(col);
           break;
@@ -1598,12 +1668,6 @@ PoaAlignKernel(const KArgs a) {
     }
     __syncthreads();
   }
-}
-
-// Zeroes the job counters of a pass (a kernel rather than a memset so the
-// launch sequence can be captured in a graph).
-__global__ void ResetCounters(uint32_t* p, uint32_t n) {
-  if (blockIdx.x == 0 && threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 }  // namespace spoa_hip_dev

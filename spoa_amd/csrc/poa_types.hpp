@@ -20,6 +20,7 @@ struct KParams {
   int32_t decay;    // linear: K*g; affine: K*e (per-lane decay of the scan)
   int32_t mpow[8][4];  // convex: M^(2^t) for t = 0..6 (row-major 2x2), [7] unused
   int32_t mseg[4];     // convex: M of one L-column segment
+  int32_t sisd;        // 1: the reference's scalar engine's end cell and traceback
 };
 
 // One (graph, read) job.  Offsets are in elements of the respective buffer.

@@ -549,29 +549,29 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
     const uint32_t len = X->read_len[X->read_first[w] + r];
     const uint32_t V = X->state[w].V;
     if (V == 0 || len == 0) continue;
-    const Shape sh = JobShape(P, len, V);
-    if (sh.width == 0) return WinFail(X, SPOA_HIP_OVERFLOW, kMsgOverflow);
-    cfg_len[sh.width == 32] = std::max(cfg_len[sh.width == 32], len);
+    const Shape sh = JobShape(P, len, V, E->sisd);
+    if (sh.width == 0) return WinFail(X, SPOA_HIP_OVERFLOW, E->msg->overflow);
+    cfg_len[sh.cfgw == 32] = std::max(cfg_len[sh.cfgw == 32], len);
   }
   for (uint32_t w : X->live) {
     const uint32_t len = X->read_len[X->read_first[w] + r];
     const uint32_t V = X->state[w].V;
     if (V == 0 || len == 0) continue;
-    const Shape sh = JobShape(P, len, V);
+    const Shape sh = JobShape(P, len, V, E->sisd);
     for (uint32_t d = 0; d < (X->strand ? 2u : 1u); ++d) {
       Plan pl{};
       pl.job = static_cast<uint32_t>(refs.size());
       pl.width = sh.width;
       pl.bits = sh.bits;
       pl.lanes = sh.lanes;
-      pl.cfg = ChooseCfg(E, cfg_len[sh.width == 32], sh.width, true);
+      pl.cfg = ChooseCfg(E, cfg_len[sh.cfgw == 32], sh.cfgw, true);
       pl.W = (len + sh.lanes - 1) / sh.lanes * sh.lanes;
       pl.stride = static_cast<uint32_t>(RoundUp(pl.W, static_cast<uint64_t>(pl.cfg.k)));
       pl.mat_bytes = JobBytes(P, V, pl.stride, sh.bits, pl.cfg);
       pl.graph_words = 0;  // the window's block is resident
       pl.seq_bytes = 0;
       pl.work = static_cast<uint64_t>(V) * len;
-      if (pl.mat_bytes > E->budget) return WinFail(X, SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+      if (pl.mat_bytes > E->budget) return WinFail(X, SPOA_HIP_NO_MEMORY, E->msg->no_memory);
       plans.push_back(pl);
       refs.push_back(JobRef{w, d});
     }
@@ -591,13 +591,13 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
     const JobRef& jr = refs[plans[x].job];
     pair_at[x + 1] = pair_at[x] + X->state[jr.window].V + X->read_len[X->read_first[jr.window] + r];
   }
-  if (pair_at[nj] > 0xffffffffull) return WinFail(X, SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+  if (pair_at[nj] > 0xffffffffull) return WinFail(X, SPOA_HIP_NO_MEMORY, E->msg->no_memory);
   const auto t_alloc0 = std::chrono::steady_clock::now();
   const uint64_t off_items = RoundUp(nj * sizeof(DevJob), kAlign);
   const uint64_t table_bytes = RoundUp(off_items + X->live.size() * sizeof(MergeItem), kAlign) + kAlign;
-  if (!X->h_table.Reserve(table_bytes) || !X->d_table.Reserve(table_bytes, kMsgNoMemory) ||
-      !X->d_res.Reserve(std::max<uint64_t>(nj, 1) * 16, kMsgNoMemory) ||
-      !X->d_pairs.Reserve(std::max<uint64_t>(pair_at[nj], 1) * 8, kMsgNoMemory)) {
+  if (!X->h_table.Reserve(table_bytes) || !X->d_table.Reserve(table_bytes, E->msg->no_memory) ||
+      !X->d_res.Reserve(std::max<uint64_t>(nj, 1) * 16, E->msg->no_memory) ||
+      !X->d_pairs.Reserve(std::max<uint64_t>(pair_at[nj], 1) * 8, E->msg->no_memory)) {
     return WinFail(X, SPOA_HIP_NO_MEMORY, g_last_error);
   }
   // passes: DP matrices within the budget; the arena is reused pass by pass
@@ -638,7 +638,7 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
         want = largest_pass;
         if (hipMalloc(&sg.p, want) != hipSuccess) {
           (void)hipGetLastError();
-          return WinFail(X, SPOA_HIP_NO_MEMORY, kMsgNoMemory);
+          return WinFail(X, SPOA_HIP_NO_MEMORY, E->msg->no_memory);
         }
       }
       sg.size = want;
@@ -882,8 +882,8 @@ int spoa_hip_windows_finish(spoa_hip_windows* X, int32_t msa) {
   }
   (void)cons_total;
   const uint64_t offs_bytes = 2 * (n + 1) * sizeof(uint64_t);
-  if (!X->h_offs.Reserve(offs_bytes) || !X->d_offs.Reserve(offs_bytes, kMsgNoMemory) ||
-      !X->d_out.Reserve(std::max<uint64_t>(total, 256), kMsgNoMemory) ||
+  if (!X->h_offs.Reserve(offs_bytes) || !X->d_offs.Reserve(offs_bytes, E->msg->no_memory) ||
+      !X->d_out.Reserve(std::max<uint64_t>(total, 256), E->msg->no_memory) ||
       !X->h_out.Reserve(std::max<uint64_t>(total, 256))) {
     return WinFail(X, SPOA_HIP_NO_MEMORY, g_last_error);
   }

@@ -13,10 +13,14 @@ namespace spoa {
 // The engine AlignmentEngine::Create returns: the reference's
 // SimdAlignmentEngine slot (src/alignment_engine.cpp:68-72) filled by the
 // HIP kernels behind include/spoa_hip.h.
+//
+// With sisd it stands in for the reference's scalar engine instead
+// (SisdAlignmentEngine, src/sisd_alignment_engine.cpp, the engine of a
+// reference built without SIMD): its results and messages.
 class HipAlignmentEngine : public AlignmentEngine {
  public:
   HipAlignmentEngine(AlignmentType type, AlignmentSubtype subtype, std::int8_t m, std::int8_t n, std::int8_t g,
-                     std::int8_t e, std::int8_t q, std::int8_t c);
+                     std::int8_t e, std::int8_t q, std::int8_t c, bool sisd = false);
   ~HipAlignmentEngine() override;
 
   void Prealloc(std::uint32_t max_sequence_len, std::uint8_t alphabet_size) override;
@@ -26,6 +30,7 @@ class HipAlignmentEngine : public AlignmentEngine {
                   std::int32_t* score = nullptr) override;
 
   spoa_hip_engine* device() const { return device_; }
+  bool sisd() const { return sisd_; }
 
   // score width (16 or 32) the range bound chose for the last Align, 0 if
   // it ran no DP
@@ -38,7 +43,20 @@ class HipAlignmentEngine : public AlignmentEngine {
   spoa_hip_engine* device_ = nullptr;
   std::vector<std::int32_t> pairs_;
   int last_width_ = 0;
+  bool sisd_ = false;
 };
+
+// AlignmentEngine::Create with the engine chosen explicitly: sisd selects
+// the reference's scalar engine's results.  Create itself passes
+// DefaultSisd().
+std::unique_ptr<AlignmentEngine> CreateEngine(AlignmentType type, std::int8_t m, std::int8_t n, std::int8_t g,
+                                              std::int8_t e, std::int8_t q, std::int8_t c, bool sisd);
+
+// The engine AlignmentEngine::Create stands in for: the reference picks it
+// when it is built (its SIMD engine when built with SSE4.1 / AVX2, else the
+// scalar one); here SPOA_HIP_ENGINE=sisd selects the scalar engine's
+// results at run time, anything else (or nothing) the SIMD engine's.
+bool DefaultSisd();
 
 // AlignmentEngine::Create's checks and normalisation
 // (src/alignment_engine.cpp:32-66): throws std::invalid_argument with the

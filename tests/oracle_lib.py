@@ -29,6 +29,8 @@ class Checker:
         f("last_error").restype = ctypes.c_char_p
         f("engine_new").restype = ctypes.c_void_p
         f("engine_new").argtypes = [ctypes.c_int] * 7
+        f("sisd_engine_new").restype = ctypes.c_void_p
+        f("sisd_engine_new").argtypes = [ctypes.c_int] * 7
         f("engine_free").argtypes = [ctypes.c_void_p]
         f("graph_new").restype = ctypes.c_void_p
         f("graph_free").argtypes = [ctypes.c_void_p]
@@ -57,12 +59,13 @@ class Checker:
     def error(self):
         return self._fn("last_error")().decode()
 
-    def engine(self, typ, m, n, g, e=None, q=None, c=None):
+    def engine(self, typ, m, n, g, e=None, q=None, c=None, sisd=False):
+        """sisd=True: the reference's scalar engine (SisdAlignmentEngine)."""
         if e is None:
             e = g
         if q is None:
             q, c = g, e
-        p = self._fn("engine_new")(typ, m, n, g, e, q, c)
+        p = self._fn("sisd_engine_new" if sisd else "engine_new")(typ, m, n, g, e, q, c)
         if not p:
             raise ValueError(self.error())
         return p
@@ -149,14 +152,16 @@ def reference():
     return Checker(REF_SO, "ref_")
 
 
-def run_window(chk, params, reads, weights=None, strand=False, calls=None):
+def run_window(chk, params, reads, weights=None, strand=False, calls=None, sisd=False):
     """Align reads in order and add each to one graph, as the reference's
     tests do (test/spoa_test.cpp:39-54).  With strand=True each read is also
     aligned reverse-complemented and the better strand is kept, as the
     reference's command line tool does (src/main.cpp:285-302).
 
+    With sisd=True the engine is the reference's scalar one.
+
     Returns (consensus, msa rows, per-call records)."""
-    eng = chk.engine(*params)
+    eng = chk.engine(*params, sisd=sisd)
     gr = chk.graph()
     recs = [] if calls is None else calls
     try:
