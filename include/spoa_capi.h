@@ -141,11 +141,13 @@ spoa_hip_engine* spoa_batch_context_device(void* ctx);
 int spoa_batch_context_set_config(void* ctx, int32_t width_bits, uint32_t max_width, int32_t cols_per_lane,
                                   int32_t waves);
 /* The autotuner (no counterpart in the reference): tries launch
- * configurations on the given windows (their first `rounds` rounds), keeps
- * the fastest per score width and installs it on the context; cache_path
- * (or NULL) names a JSON cache.  Writes per width {width, cols_per_lane,
- * waves, from_cache} into out (4 ints each) and the chosen configuration's
- * device ms into out_ms; returns the number of widths, -1 on error. */
+ * configurations on the given windows (their first `rounds` rounds, or the
+ * whole batch for a score width that those rounds do not reach), keeps the
+ * fastest per score width and installs it on the context; cache_path (or
+ * NULL) names a JSON cache.  Writes per width {width, cols_per_lane, waves,
+ * flags} into out (4 ints each; flags bit 0: from the cache, bit 1: timed
+ * on the whole batch) and the chosen configuration's device ms into
+ * out_ms; returns the number of widths, -1 on error. */
 int spoa_batch_context_autotune(void* ctx, uint32_t n_windows, const uint32_t* win_read_off, const uint64_t* read_off,
                                 const char* seqs, uint32_t rounds, const char* cache_path, int32_t* out,
                                 double* out_ms, uint32_t cap);

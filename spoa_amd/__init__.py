@@ -517,7 +517,8 @@ class BatchContext:
     def autotune(self, windows, rounds=8, cache=None):
         """Picks the launch configuration (columns per lane, waves) of each
         score width by timing candidates on these windows (their first
-        rounds); returns [{width, cols_per_lane, waves, cached, kernel_ms}]."""
+        rounds, or the whole batch for a width those rounds do not reach);
+        returns [{width, cols_per_lane, waves, cached, whole_batch, kernel_ms}]."""
         if not isinstance(windows, Windows):
             windows = Windows.from_lists(windows)
         out = np.zeros(8, dtype=np.int32)
@@ -528,7 +529,8 @@ class BatchContext:
         if n < 0:
             raise ValueError(_error())
         return [{"width": int(out[4 * i]), "cols_per_lane": int(out[4 * i + 1]), "waves": int(out[4 * i + 2]),
-                 "cached": bool(out[4 * i + 3]), "kernel_ms": float(ms[i])} for i in range(n)]
+                 "cached": bool(out[4 * i + 3] & 1), "whole_batch": bool(out[4 * i + 3] & 2),
+                 "kernel_ms": float(ms[i])} for i in range(n)]
 
     def set_config(self, width_bits, cols_per_lane, waves, max_width=0xFFFFFFFF):
         """Launch configuration of the batched launches for rows up to

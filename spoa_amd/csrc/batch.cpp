@@ -597,7 +597,6 @@ std::vector<TuneResult> Autotune(const WindowSet& ws, BatchContext& ctx, std::ui
     }
   };
   const std::uint32_t saved_rounds = opt.max_rounds;
-  opt.max_rounds = std::max<std::uint32_t>(2u, rounds);
   std::vector<TuneResult> results;
   try {
     for (int width : {16, 32}) {
@@ -627,19 +626,27 @@ std::vector<TuneResult> Autotune(const WindowSet& ws, BatchContext& ctx, std::ui
           if (std::find(cands.begin(), cands.end(), std::make_pair(k, wv)) == cands.end()) cands.push_back({k, wv});
         }
       }
+      // The first rounds decide; where they hold no job of this width (the
+      // int32-width jobs of growing windows appear only once their graphs
+      // pass the int16 bound, in the late rounds), the candidates run the
+      // whole batch instead.
       double best = -1;
-      for (const auto& cd : cands) {
-        set(width, cd.first, cd.second);
-        BatchStats st;
-        (void)RunBatched(ws, ctx, &st);
-        const double ms = width == 16 ? st.kernel_ms16 : st.kernel_ms32;
-        tr.tried.push_back({cd, ms});
-        if (ms <= 0) continue;  // no job of this width
-        if (best < 0 || ms < best) {
-          best = ms;
-          tr.k = cd.first;
-          tr.waves = cd.second;
-          tr.kernel_ms = ms;
+      for (int pass = 0; pass < 2 && best < 0; ++pass) {
+        opt.max_rounds = pass == 0 ? std::max<std::uint32_t>(2u, rounds) : 0u;
+        if (pass == 1) tr.whole_batch = true;
+        for (const auto& cd : cands) {
+          set(width, cd.first, cd.second);
+          BatchStats st;
+          (void)RunBatched(ws, ctx, &st);
+          const double ms = width == 16 ? st.kernel_ms16 : st.kernel_ms32;
+          tr.tried.push_back({cd, ms});
+          if (ms <= 0) continue;  // no job of this width
+          if (best < 0 || ms < best) {
+            best = ms;
+            tr.k = cd.first;
+            tr.waves = cd.second;
+            tr.kernel_ms = ms;
+          }
         }
       }
       if (best < 0) {

@@ -121,6 +121,7 @@ struct TuneResult {
   int k = 0, waves = 0;
   double kernel_ms = 0;      // of the chosen configuration (0: from the cache)
   bool cached = false;
+  bool whole_batch = false;  // the candidates ran the whole batch (no job of this width in the first rounds)
   std::vector<std::pair<std::pair<int, int>, double>> tried;  // ((k, waves), kernel ms)
 };
 std::vector<TuneResult> Autotune(const WindowSet& ws, BatchContext& ctx, std::uint32_t rounds,

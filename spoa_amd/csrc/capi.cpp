@@ -409,7 +409,7 @@ int spoa_batch_context_autotune(void* ctx, uint32_t n_windows, const uint32_t* w
       out[4 * n + 0] = r.width;
       out[4 * n + 1] = r.k;
       out[4 * n + 2] = r.waves;
-      out[4 * n + 3] = r.cached ? 1 : 0;
+      out[4 * n + 3] = (r.cached ? 1 : 0) | (r.whole_batch ? 2 : 0);
       out_ms[n] = r.kernel_ms;
       ++n;
     }

@@ -205,7 +205,8 @@ def test_context_reuse_and_budget_split():
 
 
 def test_autotuner_picks_configs_and_keeps_results(tmp_path):
-    """The autotuner times launch configurations per score width, installs
+    """The autotuner times launch configurations per score width (the
+    int32 width, which only late rounds reach, on the whole batch), installs
     the fastest, caches it, and the batched results do not change."""
     c = spoa_amd.CONFIGS[2]
     ws = spoa_amd.config_windows(2, seed=5, n_windows=16)
@@ -218,6 +219,10 @@ def test_autotuner_picks_configs_and_keeps_results(tmp_path):
     t16 = widths[16]
     assert t16["cols_per_lane"] in (16, 32) and 1 <= t16["waves"] <= 8  # convex int16 lanes hold 16+
     assert t16["kernel_ms"] > 0 and not t16["cached"]
+    # the int32-width jobs (graphs past the int16 bound) exist only in the
+    # late rounds: their launch is timed on the whole batch
+    t32 = widths[32]
+    assert t32["whole_batch"] and t32["kernel_ms"] > 0 and t32["cols_per_lane"] in (8, 16)
     res = ctx.run(ws)
     assert res.consensus == ref.consensus
     assert res.msa == ref.msa
