@@ -389,6 +389,20 @@ def main():
                     "gcups": pc.stats["cells"] / (pel * 1e9),
                     "kernel_ms": pc.stats["kernel_ms"], "align_calls": pc.stats["align_calls"],
                     "same_consensus_as_batched": pc.consensus[0] == res.consensus[0]}
+        # the drop-in as a multi-threaded caller of the reference runs it: one
+        # engine per host thread (made inside the timed region), a few
+        # windows per thread in order
+        nt = min(threads, n_windows)
+        per_thread = {1: 8, 2: 2, 3: 4, 4: 1}[args.config]
+        nw = min(n_windows, nt * per_thread)
+        some = windows.subset(list(range(nw)))
+        t2 = time.perf_counter()
+        pm = spoa_amd.run_windows(some, *params, mode="per_call", strand=strand, msa=True, threads=nt)
+        mel = time.perf_counter() - t2
+        per_call["threads"] = {"threads": nt, "windows": nw, "windows_per_s": nw / mel,
+                               "gcups": pm.stats["cells"] / (mel * 1e9),
+                               "same_consensus_as_batched": all(pm.consensus[w] == res.consensus[w]
+                                                                for w in range(nw))}
 
     cpu = None
     if not args.no_cpu:

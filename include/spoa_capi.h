@@ -135,6 +135,8 @@ void spoa_batch_free(void* result);
 void* spoa_batch_context_new(const spoa_batch_opts* opts);
 void spoa_batch_context_free(void* ctx);
 spoa_hip_engine* spoa_batch_context_device(void* ctx);
+/* the engine of one half of the windows (0 or 1; the batched form runs two) */
+spoa_hip_engine* spoa_batch_context_half_device(void* ctx, int32_t half);
 /* Launch configuration of the context's batched launches for rows of
  * reference width up to max_width at one score width (spoa_hip_set_config
  * on both engines); cols_per_lane 0 clears that width's overrides. */

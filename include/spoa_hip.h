@@ -219,6 +219,19 @@ uint64_t spoa_hip_memory_budget(spoa_hip_engine* engine);
  * held (they are then released first). */
 uint64_t spoa_hip_memory_in_use(spoa_hip_engine* engine);
 
+/* No counterpart in the reference (measurement): the alignment kernels'
+ * shader-clock cycles per row phase, summed over wave 0 of every job since
+ * the engine was made or last reset -- query profile, vertical pass,
+ * waiting for predecessor rows, local pass, scan, its barrier, fix-up and
+ * stores, the wait before them, end cell and boundary hand-off, its
+ * barrier, end-cell reduction, traceback, the counts of row tiles and jobs;
+ * then the traceback's parts: tile staging (cycles, count), diagonal runs
+ * (cycles, steps), general steps (cycles, count), left and up gap walks
+ * (cycles) -- 22 values.  Only a library built with SPOA_ROW_PHASES
+ * (make -C spoa_amd phases) keeps them; otherwise SPOA_HIP_INVALID_ARGUMENT.
+ * Returns the number of values written. */
+int spoa_hip_row_phases(spoa_hip_engine* engine, uint64_t* out, uint32_t n, int32_t reset);
+
 /* DP-matrix bytes one job needs (the admission footprint used by the
  * serving loop), or 0 when the job runs no DP. */
 uint64_t spoa_hip_job_footprint(spoa_hip_engine* engine, uint32_t num_nodes, uint32_t len, int32_t batch);

@@ -28,6 +28,9 @@ SW, NW, OV = 0, 1, 2
 LINEAR, AFFINE, CONVEX = 0, 1, 2
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# The library built here; SPOA_AMD_LIBRARY, read when the library is first
+# loaded, names another build of it (the row-phase timing build of
+# `make -C spoa_amd phases`).
 LIB_PATH = os.path.join(HERE, "libspoa_amd.so")
 
 _lib = None
@@ -100,16 +103,20 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    path = os.environ.get("SPOA_AMD_LIBRARY") or LIB_PATH
+    if not os.path.exists(path):
         raise RuntimeError(
             "spoa_amd: %s is missing; build it with __graft_entry__.build() "
-            "(make -C spoa_amd)" % LIB_PATH)
-    L = ctypes.CDLL(LIB_PATH)
+            "(make -C spoa_amd)" % path)
+    L = ctypes.CDLL(path)
     L.spoa_last_error.restype = ctypes.c_char_p
     L.spoa_engine_new.restype = ctypes.c_void_p
     L.spoa_engine_new.argtypes = [ctypes.c_int] * 7
     L.spoa_sisd_engine_new.restype = ctypes.c_void_p
     L.spoa_sisd_engine_new.argtypes = [ctypes.c_int] * 7
+    L.spoa_hip_row_phases.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32,
+                                      ctypes.c_int32]
+    L.spoa_hip_last_error.restype = ctypes.c_char_p
     L.spoa_engine_free.argtypes = [ctypes.c_void_p]
     L.spoa_engine_subtype.argtypes = [ctypes.c_void_p]
     L.spoa_engine_prealloc.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
@@ -139,6 +146,8 @@ def lib():
     L.spoa_batch_context_free.argtypes = [ctypes.c_void_p]
     L.spoa_batch_context_device.restype = ctypes.c_void_p
     L.spoa_batch_context_device.argtypes = [ctypes.c_void_p]
+    L.spoa_batch_context_half_device.restype = ctypes.c_void_p
+    L.spoa_batch_context_half_device.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     L.spoa_batch_consensus_all.restype = ctypes.c_int64
     L.spoa_batch_consensus_all.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_void_p]
     L.spoa_batch_context_set_config.restype = ctypes.c_int
@@ -513,6 +522,10 @@ class BatchContext:
     @property
     def device(self):
         return lib().spoa_batch_context_device(self._h)
+
+    def device_of(self, half):
+        """The C ABI engine of one half of the windows (0 or 1)."""
+        return lib().spoa_batch_context_half_device(self._h, half)
 
     def autotune(self, windows, rounds=8, cache=None):
         """Picks the launch configuration (columns per lane, waves) of each

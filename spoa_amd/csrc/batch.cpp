@@ -9,6 +9,9 @@
 #include <cstdio>
 #include <map>
 #include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
 
 #include "poa_graph.hpp"
 #include "spoa_amd_host.hpp"
@@ -463,13 +466,15 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
   return out;
 }
 
-std::vector<WindowResult> RunPerCall(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats) {
-  const auto t0 = Clock::now();
-  BatchStats st;
+namespace {
+
+// The windows w = first, first + step, ... through one engine on the
+// calling thread, in order.
+void PerCallWindows(const WindowSet& ws, const BatchOptions& opt, std::uint32_t first, std::uint32_t step,
+                    std::vector<WindowResult>& out, BatchStats& st) {
   auto engine = spoa::CreateEngine(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c, opt.sisd);
   auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(engine.get());
-  std::vector<WindowResult> out(ws.n_windows);
-  for (std::uint32_t w = 0; w < ws.n_windows; ++w) {
+  for (std::uint32_t w = first; w < ws.n_windows; w += step) {
     spoa::Graph graph;
     for (std::uint64_t k = ws.win_read_off[w]; k < ws.win_read_off[w + 1]; ++k) {
       const char* s = ws.seqs + ws.read_off[k];
@@ -518,6 +523,48 @@ std::vector<WindowResult> RunPerCall(const WindowSet& ws, const BatchOptions& op
     out[w].consensus = graph.GenerateConsensus();
     if (opt.msa) out[w].msa = graph.GenerateMultipleSequenceAlignment();
     st.finish_ms += Ms(tf, Clock::now());
+  }
+}
+
+}  // namespace
+
+std::vector<WindowResult> RunPerCall(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats) {
+  const auto t0 = Clock::now();
+  std::vector<WindowResult> out(ws.n_windows);
+  const std::uint32_t nt = std::max<std::uint32_t>(1u, std::min<std::uint32_t>(opt.threads, ws.n_windows));
+  std::vector<BatchStats> part(nt);
+  if (nt == 1) {
+    PerCallWindows(ws, opt, 0, 1, out, part[0]);
+  } else {
+    // one engine per host thread, the reference's threading model
+    std::vector<std::thread> pool;
+    std::vector<std::string> errors(nt);
+    for (std::uint32_t t = 0; t < nt; ++t) {
+      pool.emplace_back([&, t] {
+        try {
+          PerCallWindows(ws, opt, t, nt, out, part[t]);
+        } catch (const std::exception& ex) {
+          errors[t] = ex.what();
+        }
+      });
+    }
+    for (auto& th : pool) th.join();
+    for (const auto& e : errors) {
+      if (!e.empty()) throw std::invalid_argument(e);
+    }
+  }
+  BatchStats st;
+  for (const auto& p : part) {
+    st.kernel_ms += p.kernel_ms;
+    st.h2d_ms += p.h2d_ms;
+    st.d2h_ms += p.d2h_ms;
+    st.cells += p.cells;
+    st.dp_bytes += p.dp_bytes;
+    st.launches += p.launches;
+    st.align_calls += p.align_calls;
+    st.align_ms += p.align_ms;
+    st.graph_ms += p.graph_ms;
+    st.finish_ms += p.finish_ms;
   }
   st.wall_ms = Ms(t0, Clock::now());
   if (stats) *stats = st;

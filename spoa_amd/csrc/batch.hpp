@@ -128,7 +128,10 @@ std::vector<TuneResult> Autotune(const WindowSet& ws, BatchContext& ctx, std::ui
                                  const std::string& cache_path);
 
 // Plain per-call form: AlignmentEngine::Create(...)->Align for every read
-// of every window, in order, on the calling thread.
+// of every window, in order; on the calling thread, or with
+// opt.threads > 1 on that many host threads with an engine each (windows
+// dealt round-robin), as a multi-threaded caller of the reference runs.
+// The stats' times are summed over the threads (wall_ms is the run's).
 std::vector<WindowResult> RunPerCall(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats);
 
 std::string ReverseComplement(const char* s, std::size_t len);

@@ -378,6 +378,12 @@ spoa_hip_engine* spoa_batch_context_device(void* ctx) {
   return hip ? hip->device() : nullptr;
 }
 
+spoa_hip_engine* spoa_batch_context_half_device(void* ctx, int32_t half) {
+  if (half < 0 || half > 1) return nullptr;
+  auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(static_cast<spoa_amd::BatchContext*>(ctx)->engine(half));
+  return hip ? hip->device() : nullptr;
+}
+
 int spoa_batch_context_set_config(void* ctx, int32_t width_bits, uint32_t max_width, int32_t cols_per_lane,
                                   int32_t waves) {
   auto* c = static_cast<spoa_amd::BatchContext*>(ctx);

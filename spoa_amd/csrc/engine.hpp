@@ -299,6 +299,9 @@ struct spoa_hip_engine {
   // SIMD engine's (spoa_hip_engine_create_sisd)
   bool sisd = false;
   const spoa_hip_host::EngineMsgs* msg = &spoa_hip_host::kSimdMsgs;
+  // row-phase cycle counters of the alignment kernels (builds with
+  // SPOA_ROW_PHASES; null otherwise)
+  unsigned long long* phases = nullptr;
   int num_cus = 256;
   uint64_t budget = 0;
   DevBuf d_in, d_out, d_counter;
@@ -398,8 +401,12 @@ struct LaunchGroup {
 bool LaunchResetCounters(uint32_t* counters, uint32_t n, hipStream_t stream);
 
 // Places matrices of the given sizes in the engine's arena (first fit,
-// growing it within the budget); defined in poa_hip.hip.
-int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::vector<uint64_t>* addr);
+// growing it within the budget by segments of at least min_segment bytes);
+// defined in poa_hip.hip.
+constexpr uint64_t kBatchSegment = 4ull << 30;      // batched passes
+constexpr uint64_t kPerCallSegment = 64ull << 20;   // one alignment per call
+int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::vector<uint64_t>* addr,
+                  uint64_t min_segment);
 
 // Launch groups of plans[lo, hi) (sorted by group, largest V * n first in
 // one); defined in poa_hip.hip.

@@ -77,7 +77,8 @@ bool LaunchResetCounters(uint32_t* counters, uint32_t n, hipStream_t stream) {
 // doubling it, so that batches whose jobs grow round by round allocate
 // O(log) times); when the segments are too fragmented, starts over with one
 // segment holding everything.  Device addresses into addr.
-int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::vector<uint64_t>* addr_out) {
+int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::vector<uint64_t>* addr_out,
+                  uint64_t min_segment) {
   const size_t nj = need.size();
   std::vector<uint64_t>& addr = *addr_out;
   addr.assign(nj, 0);
@@ -101,7 +102,7 @@ int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::ve
     uint64_t rest = 0;
     for (size_t y = x; y < nj; ++y) rest += need[y];
     const uint64_t room = E->budget > E->arena_bytes ? E->budget - E->arena_bytes : 0;
-    uint64_t want = std::min(room, std::max<uint64_t>(std::max<uint64_t>(rest, 4ull << 30), E->arena_bytes));
+    uint64_t want = std::min(room, std::max<uint64_t>(std::max<uint64_t>(rest, min_segment), E->arena_bytes));
     if (want < need[x]) {
       // fragmented: start over with one segment of twice what the whole
       // set needs (within the budget), so that sets which grow round by
@@ -115,7 +116,7 @@ int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::ve
       uint64_t all = 0;
       for (uint64_t b : need) all += b;
       DevBuf sg;
-      const uint64_t one = std::min<uint64_t>(E->budget, std::max<uint64_t>(2 * all, 4ull << 30));
+      const uint64_t one = std::min<uint64_t>(E->budget, std::max<uint64_t>(2 * all, min_segment));
       if (one >= all && hipMalloc(&sg.p, one) == hipSuccess) {
         sg.size = one;
         E->arena.push_back(sg);
@@ -525,7 +526,11 @@ void ParallelRanges(spoa_hip_engine* E, size_t n, const Fn& fn) {
 // transfer on the engine's copy stream, aligned by one persistent launch
 // per group on the compute stream (one after another, each with the device
 // to itself and timed by its own events) and copied back in one transfer.
-int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& plans_in, size_t lo, size_t hi) {
+// batch: a pass of the batched form (its arena starts at 4 GiB segments);
+// else one per-call alignment (64 MiB, doubling as the graphs grow), so that
+// an engine per host thread does not hold gigabytes for its first reads.
+int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& plans_in, size_t lo, size_t hi,
+                  bool batch) {
   auto& pd = E->pending;
   pd.plans = std::move(plans_in);
   const std::vector<Plan>& plans = pd.plans;
@@ -561,7 +566,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   {
     std::vector<uint64_t> need(nj);
     for (size_t x = 0; x < nj; ++x) need[x] = plans[lo + x].mat_bytes;
-    const int rc = PlaceMatrices(E, need, &addr);
+    const int rc = PlaceMatrices(E, need, &addr, batch ? kBatchSegment : kPerCallSegment);
     if (rc != SPOA_HIP_OK) return rc;
   }
   E->stats.alloc_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_alloc0).count();
@@ -662,6 +667,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     args.results = reinterpret_cast<int32_t*>(dout + off_res) + 4 * gp.a;
     args.lds_bytes = gp.lds;
     args.prio_jobs = gp.prio;
+    args.phases = E->phases;
     return args;
   };
   E->pass_width.clear();
@@ -961,7 +967,7 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     const size_t hi = spoa_amd::PassEnd(items.data(), lo, items.size(), E->budget);
     const bool last = hi == plans.size();
     // the pass owns a copy of the plans while it is in flight
-    int rc = RunPassLaunch(E, jobs, std::vector<Plan>(plans), lo, hi);
+    int rc = RunPassLaunch(E, jobs, std::vector<Plan>(plans), lo, hi, batch);
     if (rc == SPOA_HIP_OK && !(last && async)) rc = RunPassFinish(E);
     if (rc != SPOA_HIP_OK && first == SPOA_HIP_OK) first = rc;
     if (rc == SPOA_HIP_DEVICE_ERROR || rc == SPOA_HIP_NO_MEMORY || rc == SPOA_HIP_INVALID_ARGUMENT) {
@@ -1019,8 +1025,31 @@ int spoa_hip_engine_create(int device, const spoa_hip_params* params, spoa_hip_e
   }
   E->compute = E->stream;
   E->copy = E->stream;
+#ifdef SPOA_ROW_PHASES
+  if (!HipOk(hipMalloc(&E->phases, 64 * sizeof(unsigned long long)), "hipMalloc") ||
+      !HipOk(hipMemset(E->phases, 0, 64 * sizeof(unsigned long long)), "hipMemset")) {
+    spoa_hip_engine_destroy(E);
+    return SPOA_HIP_DEVICE_ERROR;
+  }
+#endif
   *out = E;
   return SPOA_HIP_OK;
+}
+
+int spoa_hip_row_phases(spoa_hip_engine* E, uint64_t* out, uint32_t n, int32_t reset) {
+  if (!E || (!out && n)) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null argument");
+  if (!E->phases) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] built without row-phase timing");
+  if (!HipOk(hipSetDevice(E->device), "hipSetDevice") || !HipOk(hipDeviceSynchronize(), "hipDeviceSynchronize")) {
+    return SPOA_HIP_DEVICE_ERROR;
+  }
+  const uint32_t k = std::min<uint32_t>(n, spoa_hip_dev::kPhTotal);
+  if (k && !HipOk(hipMemcpy(out, E->phases, k * sizeof(uint64_t), hipMemcpyDeviceToHost), "copy out")) {
+    return SPOA_HIP_DEVICE_ERROR;
+  }
+  if (reset && !HipOk(hipMemset(E->phases, 0, 64 * sizeof(unsigned long long)), "hipMemset")) {
+    return SPOA_HIP_DEVICE_ERROR;
+  }
+  return static_cast<int>(k);
 }
 
 int spoa_hip_engine_create_sisd(int device, const spoa_hip_params* params, spoa_hip_engine** out) {
@@ -1064,6 +1093,7 @@ int spoa_hip_engine_destroy(spoa_hip_engine* E) {
     (void)hipStreamDestroy(E->compute_own);
   }
   if (E->stream) (void)hipStreamDestroy(E->stream);
+  if (E->phases) (void)hipFree(E->phases);
   delete E;
   return SPOA_HIP_OK;
 }
@@ -1206,6 +1236,7 @@ int spoa_hip_align_noalloc(spoa_hip_engine* E, const spoa_hip_graph* graph, cons
   const int threads = 64 * cfg.waves;
   args.lds_bytes = 8u * static_cast<uint32_t>(cfg.k) * static_cast<uint32_t>(threads);
   args.prio_jobs = 0;
+  args.phases = E->phases;
   fn<<<1, threads, args.lds_bytes, st>>>(args);
   if (!HipOk(hipGetLastError(), "kernel launch")) return SPOA_HIP_DEVICE_ERROR;
   spoa_hip_dev::NoAllocFinish<<<1, 256, 0, st>>>(pa.res, pairs, cap, sh.width, result);

@@ -102,6 +102,41 @@ __device__ __forceinline__ void WaitVmem() { __builtin_amdgcn_s_waitcnt(0x0F70);
 // for its global loads and stores (which __syncthreads would also drain).
 __device__ __forceinline__ void LdsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// Row-phase timing (a build with SPOA_ROW_PHASES only): wave 0 of every
+// workgroup reads the shader clock (s_memtime) at the phase boundaries of
+// each row tile and adds the cycles of each phase into KArgs::phases at
+// the end of each job.  Phases (the RowPhase order): query profile,
+// vertical pass (of which: waiting for predecessor loads), local pass,
+// scan (of which: the wave-total barrier), fix-up and stores (of which:
+// the wait before the first store), end cell and boundary hand-off (of
+// which: its barrier); then per job: end-cell reduction and traceback;
+// and counts of row tiles and jobs.
+enum RowPhase {
+  kPhProfile = 0, kPhVertical, kPhPredWait, kPhLocal, kPhScan, kPhScanBarrier, kPhFixup, kPhStoreWait,
+  kPhEndBoundary, kPhBoundaryBarrier, kPhReduce, kPhTraceback, kPhTiles, kPhJobs, kPhCount
+};
+// Traceback parts, after the row phases: tile staging (cycles, count),
+// diagonal runs (cycles, steps), general steps (cycles, count), left and up
+// gap walks (cycles).
+enum TbPhase { kTbStage = 0, kTbStages, kTbDiag, kTbDiagSteps, kTbGeneral, kTbGenerals, kTbLeft, kTbUp, kTbCount };
+constexpr int kPhTotal = kPhCount + kTbCount;
+#ifdef SPOA_ROW_PHASES
+// the scheduler may not move work across a timestamp
+#define SPOA_PH_NOW()                                    \
+  ({                                                     \
+    __builtin_amdgcn_sched_barrier(0);                   \
+    unsigned long long t_ = __builtin_amdgcn_s_memtime(); \
+    __builtin_amdgcn_sched_barrier(0);                   \
+    t_;                                                  \
+  })
+#define SPOA_PH_ADD(k, v) (ph[k] += (v))
+#define SPOA_TB_ADD(k, v) (tph[k] += (v))
+#else
+#define SPOA_PH_NOW() 0ull
+#define SPOA_PH_ADD(k, v) ((void)0)
+#define SPOA_TB_ADD(k, v) ((void)0)
+#endif
+
 __device__ __forceinline__ uint32_t Uniform(uint32_t v) {
   return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
 }
@@ -161,7 +196,9 @@ template <int SUB, typename ST>
 __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t* __restrict__ seq,
                           uint32_t n, const ST* __restrict__ mat, uint64_t rstride, uint32_t stride,
                           int32_t max_i, int32_t max_j, int32_t* __restrict__ out, uint32_t cap,
-                          uint32_t* n_out, int32_t* status_out, Lds<uint32_t>* lds, uint32_t lds_bytes, int lane) {
+                          uint32_t* n_out, int32_t* status_out, Lds<uint32_t>* lds, uint32_t lds_bytes, int lane,
+                          unsigned long long (&tph)[kTbCount]) {
+  (void)tph;
   constexpr int NM = SUB == 0 ? 1 : (SUB == 1 ? 3 : 5);
   constexpr int S = sizeof(ST);
   constexpr int VEC = 16 / S;  // columns a 16-byte load
@@ -213,6 +250,8 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
   }
   // stage the tile whose top-right cell is (i, j + 1)
   auto stage = [&](int32_t i, int32_t j) {
+    const unsigned long long t0 = SPOA_PH_NOW();
+    (void)t0;
     T.r1 = i;
     T.r0 = max(0, i - T.R + 1);
     int32_t c0 = j + 2 - T.C;
@@ -257,6 +296,8 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     // the wave's LDS writes land before any lane reads them
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
+    SPOA_TB_ADD(kTbStage, SPOA_PH_NOW() - t0);
+    SPOA_TB_ADD(kTbStages, 1);
   };
   auto ensure = [&](int32_t i, int32_t j) {
     if (!T.on) return;
@@ -365,6 +406,8 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     // diagonal test below with k = 0 and j >= 1): each step reads the
     // predecessor's cell, its record and the next base in one round.
     if (T.on && i >= T.rb && i <= T.r1 && j >= T.c0 + 1 && j < T.c0 + T.C) {
+      const unsigned long long td0 = SPOA_PH_NOW();
+      (void)td0;
       bool moved = false;
       int32_t h = at(0, i, j);
       U32x4 rc = T.rec[i - T.rb];
@@ -387,10 +430,14 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
         rc = rn;
         p0 = U(static_cast<int32_t>(rc.x));
         moved = true;
+        SPOA_TB_ADD(kTbDiagSteps, 1);
         if (i == 0 || i < T.rb) break;
       }
+      SPOA_TB_ADD(kTbDiag, SPOA_PH_NOW() - td0);
       if (moved) continue;
     }
+    const unsigned long long tg0 = SPOA_PH_NOW();
+    (void)tg0;
     // The general step: one lane a predecessor, 64 at a time; the first
     // hit in predecessor order wins, as in the serial loops.
     const int32_t h = cell(0, i, j);
@@ -472,6 +519,10 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     emit(i == prev_i ? -1 : node_of(i), j == prev_j ? -1 : j);
     i = prev_i;
     j = prev_j;
+    const unsigned long long tg1 = SPOA_PH_NOW();
+    (void)tg1;
+    SPOA_TB_ADD(kTbGeneral, tg1 - tg0);
+    SPOA_TB_ADD(kTbGenerals, 1);
 
     if (ext_left) {
       // :1484-1509, :2006-2039 -- convex stops when either chain breaks
@@ -507,6 +558,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
         }
       }
       j = jj;
+      SPOA_TB_ADD(kTbLeft, SPOA_PH_NOW() - tg1);
     } else if (ext_up) {
       // :1510-1547 affine, :2040-2093 convex
       while (i != 0 && status == kDevOk) {
@@ -565,6 +617,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
         i = prev_i;
         if (stop || i == 0) break;
       }
+      SPOA_TB_ADD(kTbUp, SPOA_PH_NOW() - tg1);
     }
   }
 
@@ -802,6 +855,9 @@ PoaAlignKernel(const KArgs a) {
   static_assert(SUB != 2 || K % L == 0, "convex lanes must start reference segments");
   static_assert(KW % 4 == 0 && PW % 4 == 0, "rows move in 16-byte accesses");
   constexpr int32_t kMinI = -2147483647 - 1;
+#ifdef SPOA_ROW_PHASES
+  unsigned long long ph[kPhCount] = {};
+#endif
 
   __shared__ int32_t s_tot[2][kMaxWaves][2];
   __shared__ int32_t s_bnd[kMaxWaves];
@@ -987,6 +1043,7 @@ PoaAlignKernel(const KArgs a) {
         const uint32_t c0 = t * tile_w + static_cast<uint32_t>(tid) * K;
         const bool active = c0 < stride;
         const uint32_t off = c0 * S;  // byte offset of the lane's columns in a matrix
+        const unsigned long long tm0 = SPOA_PH_NOW();
 
         // ---- query profile of the row base
         if (!single) LoadChars<K>(seq, c0, n, active, scp, nvalid);
@@ -1005,6 +1062,7 @@ PoaAlignKernel(const KArgs a) {
           ProfileWords<K>(scp, nvalid, rch, m, mis, pad, pw);
         }
 
+        const unsigned long long tm1 = SPOA_PH_NOW();
         // ---- 1. vertical pass over the predecessors, in in-edge order
         uint32_t h1[KW], fv[KW], ov[KW];
         for (uint32_t pi = 0; pi < npred; ++pi) {
@@ -1037,7 +1095,9 @@ PoaAlignKernel(const KArgs a) {
             if (lane == 0 && c0 != 0 && p != 0) {
               hb = bnd[static_cast<uint64_t>(t * static_cast<uint32_t>(nwaves) + wave) * (V + 1) + p];
             }
+            const unsigned long long tw0 = SPOA_PH_NOW();
             WaitVmem();
+            SPOA_PH_ADD(kPhPredWait, SPOA_PH_NOW() - tw0);
           }
           // H_p[c0 - 1]: neighbouring lane, or the first column / memory
           // across waves and tiles
@@ -1154,6 +1214,7 @@ PoaAlignKernel(const KArgs a) {
           }
         }
 
+        const unsigned long long tm2 = SPOA_PH_NOW();
         // ---- 2. local pass: the horizontal gaps inside the lane with the
         // carry absent.  loc holds, per column, the best horizontal value
         // reachable from inside the lane (linear H, affine E, convex (E, Q)),
@@ -1253,6 +1314,7 @@ PoaAlignKernel(const KArgs a) {
           }
         }
 
+        const unsigned long long tm3 = SPOA_PH_NOW();
         // ---- wavefront inclusive scan of the lane transfers.  Inside each
         // row of 16 lanes a Kogge-Stone scan on DPP row shifts, the sender
         // applying the uniform step transfer M^sh (scalar: sh times the lane
@@ -1323,7 +1385,9 @@ PoaAlignKernel(const KArgs a) {
             s_tot[par][wave][0] = s0;
             s_tot[par][wave][1] = s1;
           }
+          const unsigned long long tb0 = SPOA_PH_NOW();
           LdsBarrier();
+          SPOA_PH_ADD(kPhScanBarrier, SPOA_PH_NOW() - tb0);
           int32_t f0 = y0, f1 = y1;
           for (int w = 0; w < nwaves; ++w) {
             if (w == wave) {
@@ -1362,13 +1426,16 @@ PoaAlignKernel(const KArgs a) {
           }
         }
 
+        const unsigned long long tm4 = SPOA_PH_NOW();
         // ---- 3. fix-up: every column's value is the better of its local
         // value and the carry moved to it; then H.  E / Q leave in 16-byte
         // chunks as soon as a chunk is complete, H after the row.  The
         // prefetch has had the row's work to land; it is waited for here,
         // before the row's first store (F, O), so that the next row never
         // waits for this row's stores.
+        const unsigned long long ts0 = SPOA_PH_NOW();
         WaitVmem();
+        SPOA_PH_ADD(kPhStoreWait, SPOA_PH_NOW() - ts0);
         if (SUB >= 1 && active) {
           BufStore<KW>(rrow, off + mbytes, fv);
           if (SUB == 2) BufStore<KW>(rrow, off + 3 * mbytes, ov);
@@ -1496,6 +1563,7 @@ PoaAlignKernel(const KArgs a) {
           for (int i = 0; i < KW; ++i) cH[i] = hw[i];
         }
 
+        const unsigned long long tm5 = SPOA_PH_NOW();
         // end-cell candidates (:903-928): SW all rows, OV sink rows (row
         // maxima floored at 0, :252-262), NW sink rows at column n - 1.
         // The scalar engine takes the first strictly better cell in
@@ -1556,7 +1624,9 @@ PoaAlignKernel(const KArgs a) {
           bool have = false;
           if (nwaves > 1) {
             if (lane == 63) s_bnd[wave] = last;
+            const unsigned long long tb1 = SPOA_PH_NOW();
             LdsBarrier();
+            SPOA_PH_ADD(kPhBoundaryBarrier, SPOA_PH_NOW() - tb1);
             if (lane == 0) {
               if (wave > 0) {
                 got = s_bnd[wave - 1];
@@ -1580,9 +1650,24 @@ PoaAlignKernel(const KArgs a) {
             if (single) bprev = got;
           }
         }
+#ifdef SPOA_ROW_PHASES
+        {
+          const unsigned long long tm6 = SPOA_PH_NOW();
+          ph[kPhProfile] += tm1 - tm0;
+          ph[kPhVertical] += tm2 - tm1;
+          ph[kPhLocal] += tm3 - tm2;
+          ph[kPhScan] += tm4 - tm3;
+          ph[kPhFixup] += tm5 - tm4;
+          ph[kPhEndBoundary] += tm6 - tm5;
+          ph[kPhTiles] += 1;
+        }
+#else
+        (void)tm0, (void)tm1, (void)tm2, (void)tm3, (void)tm4, (void)tm5;
+#endif
       }
       fch_prev = fch_r;
     }
+    const unsigned long long tr0 = SPOA_PH_NOW();
     // the traceback reads what every wave stored
     __threadfence_block();
 
@@ -1642,6 +1727,9 @@ PoaAlignKernel(const KArgs a) {
       __syncthreads();
     }
 
+    const unsigned long long tr1 = SPOA_PH_NOW();
+    SPOA_PH_ADD(kPhReduce, tr1 - tr0);
+    (void)tr0, (void)tr1;
     // ---- traceback by wave 0 (the query profile's LDS is free now)
     if (wave == 0) {
       int32_t* res = a.results + 4 * jx;
@@ -1655,9 +1743,18 @@ PoaAlignKernel(const KArgs a) {
       } else {
         uint32_t np = 0;
         int32_t status = kDevOk;
+        const unsigned long long tt0 = SPOA_PH_NOW();
+        unsigned long long tph[kTbCount] = {};
         Traceback<SUB, ST>(prm, gv, seq, n, mat, rstride, stride, max_i, max_j, a.pairs + 2ull * J.pair_off,
                            J.cap, &np, &status, (Lds<uint32_t>*)(s_prof),
-                           a.lds_bytes, lane);
+                           a.lds_bytes, lane, tph);
+#ifdef SPOA_ROW_PHASES
+        if (lane == 0 && a.phases) {
+          for (int k = 0; k < kTbCount; ++k) atomicAdd(a.phases + kPhCount + k, tph[k]);
+        }
+#endif
+        SPOA_PH_ADD(kPhTraceback, SPOA_PH_NOW() - tt0);
+        (void)tt0;
         if (lane == 0) {
           res[0] = static_cast<int32_t>(np);
           res[1] = max_score;
@@ -1666,6 +1763,15 @@ PoaAlignKernel(const KArgs a) {
         }
       }
     }
+#ifdef SPOA_ROW_PHASES
+    if (wave == 0) {
+      ph[kPhJobs] += 1;
+      if (lane == 0 && a.phases) {
+        for (int k = 0; k < kPhCount; ++k) atomicAdd(a.phases + k, ph[k]);
+      }
+    }
+    for (int k = 0; k < kPhCount; ++k) ph[k] = 0;
+#endif
     __syncthreads();
   }
 }

@@ -94,6 +94,7 @@ struct KArgs {
   int32_t* results;          // per job: n_pairs, score, has_score, status
   uint32_t lds_bytes;        // dynamic LDS of the launch (at least the query profile's)
   uint32_t prio_jobs;        // the first prio_jobs jobs of the queue run at raised wave priority
+  unsigned long long* phases;  // row-phase cycle counters (builds with SPOA_ROW_PHASES; else unused)
 };
 
 enum DevStatus : int32_t { kDevOk = 0, kDevCapacity = 6, kDevInconsistent = 7 };

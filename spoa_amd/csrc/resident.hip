@@ -653,7 +653,7 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
     const size_t lo = passes[pi].first, hi = passes[pi].second;
     std::vector<uint64_t> need(hi - lo);
     for (size_t x = lo; x < hi; ++x) need[x - lo] = plans[x].mat_bytes;
-    const int rc = PlaceMatrices(E, need, &pass_addr[pi]);
+    const int rc = PlaceMatrices(E, need, &pass_addr[pi], kBatchSegment);
     if (rc != SPOA_HIP_OK) return WinFail(X, rc, g_last_error);
     for (size_t x = lo; x < hi; ++x) {
       const Plan& pl = plans[x];
@@ -733,6 +733,7 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
       args.results = static_cast<int32_t*>(X->d_res.p) + 4 * (lo + gp.a);
       args.lds_bytes = gp.lds;
       args.prio_jobs = gp.prio;
+      args.phases = E->phases;
       if (!HipOk(hipEventRecord(X->ev_s[X->launches], st), "event")) return SPOA_HIP_DEVICE_ERROR;
       gp.fn<<<gp.grid, gp.threads, gp.lds, st>>>(args);
       if (!HipOk(hipGetLastError(), "kernel launch") || !HipOk(hipEventRecord(X->ev_e[X->launches], st), "event")) {

@@ -250,3 +250,15 @@ def test_graph_mode_replays_match_stream_mode():
         assert res.msa == ref.msa
         again = eager.run(ws)
         assert again.consensus == ref.consensus
+
+
+def test_per_call_one_engine_per_host_thread():
+    """The per-call drop-in on several host threads, an engine each (the
+    reference's threading model): the same windows as the batched form."""
+    c = spoa_amd.CONFIGS[3]
+    ws = spoa_amd.config_windows(3, seed=12, n_windows=8)
+    ref = spoa_amd.run_windows(ws, *c["params"], mode="batched", msa=True)
+    res = spoa_amd.run_windows(ws, *c["params"], mode="per_call", msa=True, threads=4)
+    assert res.consensus == ref.consensus
+    assert res.msa == ref.msa
+
