@@ -142,6 +142,10 @@ spoa_hip_engine* spoa_batch_context_half_device(void* ctx, int32_t half);
  * on both engines); cols_per_lane 0 clears that width's overrides. */
 int spoa_batch_context_set_config(void* ctx, int32_t width_bits, uint32_t max_width, int32_t cols_per_lane,
                                   int32_t waves);
+/* The same with split (spoa_hip_set_config_split): rows that span several
+ * tiles run each tile on a workgroup of its own. */
+int spoa_batch_context_set_config_split(void* ctx, int32_t width_bits, uint32_t max_width, int32_t cols_per_lane,
+                                        int32_t waves, int32_t split);
 /* The autotuner (no counterpart in the reference): tries launch
  * configurations on the given windows (their first `rounds` rounds, or the
  * whole batch for a score width that those rounds do not reach), keeps the

@@ -188,6 +188,24 @@ int spoa_hip_config(spoa_hip_engine* engine, uint32_t len, int32_t width_bits, i
  * and mode. */
 int spoa_hip_set_config(spoa_hip_engine* engine, int32_t width_bits, int32_t batch, uint32_t max_width,
                         int32_t cols_per_lane, int32_t waves);
+/* No counterpart in the reference.  As spoa_hip_config / spoa_hip_set_config,
+ * with split: != 0 when a job whose row spans several tiles of waves x 64 x
+ * cols_per_lane columns runs every tile on a workgroup of its own (each
+ * tile hands every row's boundary -- its last H and the gap carry -- to the
+ * next tile through device memory, the tiles of a job running side by
+ * side), 0 when one workgroup runs a row's tiles in turn.  By default rows
+ * wider than 8 waves are split into one-wave tiles.  spoa_hip_set_config
+ * sets split = 0. */
+int spoa_hip_config_split(spoa_hip_engine* engine, uint32_t len, int32_t width_bits, int32_t batch,
+                          int32_t* cols_per_lane, int32_t* waves, int32_t* split);
+int spoa_hip_set_config_split(spoa_hip_engine* engine, int32_t width_bits, int32_t batch, uint32_t max_width,
+                              int32_t cols_per_lane, int32_t waves, int32_t split);
+/* No counterpart in the reference.  The longest time one row tile of a
+ * split job waits for another, in ticks of the device's 100 MHz clock
+ * (default 10^9, 10 s); a job whose tiles wait longer fails with
+ * SPOA_HIP_DEVICE_ERROR ("... timed out waiting for each other") and the
+ * launch's other split jobs stop waiting.  For tests of that path. */
+int spoa_hip_set_wait_limit(spoa_hip_engine* engine, uint32_t ticks);
 
 /* No counterpart in the reference.  Makes engine launch its kernels on
  * owner's compute stream (owner NULL: back on its own), so the kernels of

@@ -153,6 +153,9 @@ def lib():
     L.spoa_batch_context_set_config.restype = ctypes.c_int
     L.spoa_batch_context_set_config.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int32,
                                                 ctypes.c_int32]
+    L.spoa_batch_context_set_config_split.restype = ctypes.c_int
+    L.spoa_batch_context_set_config_split.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32,
+                                                      ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     L.spoa_batch_context_autotune.restype = ctypes.c_int
     L.spoa_batch_context_autotune.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_void_p,
@@ -196,6 +199,11 @@ def lib():
                                       ctypes.c_int32, ctypes.c_int32]
     L.spoa_hip_config.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32,
                                   _i32p, _i32p]
+    L.spoa_hip_set_config_split.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
+                                            ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.spoa_hip_config_split.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32,
+                                        _i32p, _i32p, _i32p]
+    L.spoa_hip_set_wait_limit.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
     L.spoa_hip_set_memory_budget.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
     L.spoa_hip_memory_budget.restype = ctypes.c_uint64
     L.spoa_hip_memory_budget.argtypes = [ctypes.c_void_p]
@@ -545,10 +553,12 @@ class BatchContext:
                  "cached": bool(out[4 * i + 3] & 1), "whole_batch": bool(out[4 * i + 3] & 2),
                  "kernel_ms": float(ms[i])} for i in range(n)]
 
-    def set_config(self, width_bits, cols_per_lane, waves, max_width=0xFFFFFFFF):
+    def set_config(self, width_bits, cols_per_lane, waves, max_width=0xFFFFFFFF, split=False):
         """Launch configuration of the batched launches for rows up to
-        max_width columns at one score width; cols_per_lane 0 clears."""
-        if lib().spoa_batch_context_set_config(self._h, width_bits, max_width, cols_per_lane, waves) != 0:
+        max_width columns at one score width; cols_per_lane 0 clears.  With
+        split, a row of several tiles runs each tile on its own workgroup."""
+        if lib().spoa_batch_context_set_config_split(self._h, width_bits, max_width, cols_per_lane, waves,
+                                                     1 if split else 0) != 0:
             raise ValueError(_error())
 
     def run(self, windows):

@@ -384,18 +384,24 @@ spoa_hip_engine* spoa_batch_context_half_device(void* ctx, int32_t half) {
   return hip ? hip->device() : nullptr;
 }
 
-int spoa_batch_context_set_config(void* ctx, int32_t width_bits, uint32_t max_width, int32_t cols_per_lane,
-                                  int32_t waves) {
+int spoa_batch_context_set_config_split(void* ctx, int32_t width_bits, uint32_t max_width, int32_t cols_per_lane,
+                                        int32_t waves, int32_t split) {
   auto* c = static_cast<spoa_amd::BatchContext*>(ctx);
   for (int h = 0; h < 2; ++h) {
     auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(c->engine(h));
     if (!hip) return -1;
-    if (spoa_hip_set_config(hip->device(), width_bits, 1, max_width, cols_per_lane, waves) != SPOA_HIP_OK) {
+    if (spoa_hip_set_config_split(hip->device(), width_bits, 1, max_width, cols_per_lane, waves, split) !=
+        SPOA_HIP_OK) {
       g_error = spoa_hip_last_error();
       return -1;
     }
   }
   return 0;
+}
+
+int spoa_batch_context_set_config(void* ctx, int32_t width_bits, uint32_t max_width, int32_t cols_per_lane,
+                                  int32_t waves) {
+  return spoa_batch_context_set_config_split(ctx, width_bits, max_width, cols_per_lane, waves, 0);
 }
 
 int spoa_batch_context_autotune(void* ctx, uint32_t n_windows, const uint32_t* win_read_off, const uint64_t* read_off,

@@ -129,11 +129,35 @@ inline Shape JobShape(const spoa_hip_params& p, uint32_t len, uint32_t V, bool s
 inline int Matrices(int subtype) { return subtype == 0 ? 1 : (subtype == 1 ? 3 : 5); }
 inline int ReadMatrices(int subtype) { return subtype == 0 ? 1 : (subtype == 1 ? 2 : 3); }
 
+// Launch configuration: columns per lane, waves per workgroup, and whether
+// a job whose row spans several tiles of waves x 64 x k columns runs each
+// tile on its own workgroup (split) or all of them in turn on one.
 struct Cfg {
   int k = 8;
   int waves = 1;
-  bool operator<(const Cfg& o) const { return std::tie(k, waves) < std::tie(o.k, o.waves); }
+  int split = 0;
+  bool operator<(const Cfg& o) const { return std::tie(k, waves, split) < std::tie(o.k, o.waves, o.split); }
 };
+
+// Row tiles of a job with row stride `stride` under cfg that run on
+// workgroups of their own: 1 unless the job is split.
+inline uint32_t JobTiles(uint32_t stride, const Cfg& cfg) {
+  if (!cfg.split) return 1;
+  const uint32_t tile = 64u * static_cast<uint32_t>(cfg.k) * static_cast<uint32_t>(cfg.waves);
+  return (stride + tile - 1) / tile;
+}
+
+// A split job's 64-bit hand-off words (spoa_hip_dev::DevJob): three per row
+// and tile boundary, two per tile.
+inline uint64_t XferWords(uint32_t V, uint32_t tiles) {
+  if (tiles <= 1) return 0;
+  const uint64_t w = 3ull * (tiles - 1) * (static_cast<uint64_t>(V) + 1) + 2ull * tiles;
+  return (w + 1) / 2 * 2;
+}
+
+// Jobs are split into at most this many tiles (longer rows run their tiles
+// in turn).
+constexpr uint32_t kMaxSplitTiles = 4096;
 
 // Device buffer, grow-only.
 struct DevBuf {
@@ -261,6 +285,7 @@ struct Plan {
   uint64_t graph_words;
   uint64_t seq_bytes;
   uint64_t work;  // V * n: the queue order inside a launch group
+  uint32_t tiles = 1;  // workgroups of a split job (queue items), else 1
 };
 
 struct spoa_hip_engine {
@@ -305,6 +330,12 @@ struct spoa_hip_engine {
   int num_cus = 256;
   uint64_t budget = 0;
   DevBuf d_in, d_out, d_counter;
+  // split jobs: their hand-off words (zeroed when allocated; each launch
+  // tags its words with a new epoch), and the longest wait of one tile for
+  // another (100 MHz clock ticks) before the job reports a time-out
+  DevBuf d_xfer;
+  uint32_t epoch = 0;
+  uint32_t wait_ticks = 1000000000u;
   // DP matrices: grow-only segments kept for the engine's lifetime, so a
   // long-lived engine allocates once (allocation time grows with size)
   std::vector<DevBuf> arena;
@@ -343,6 +374,13 @@ inline Cfg DefaultCfg(const spoa_hip_engine* E, uint32_t W, int width_bits, bool
   const uint32_t per_wave = 64u * static_cast<uint32_t>(c.k);
   c.waves = static_cast<int>(
       std::min<uint32_t>(kMaxGroupWaves, std::max<uint32_t>(1u, (W + per_wave - 1) / per_wave)));
+  // Rows wider than one workgroup of 8 waves: every one-wave tile of the
+  // row on its own workgroup (no barriers inside a tile, and the tiles of
+  // one job run side by side on as many SIMDs).
+  if (W > per_wave * kMaxGroupWaves && (W + per_wave - 1) / per_wave <= kMaxSplitTiles) {
+    c.waves = 1;
+    c.split = 1;
+  }
   return c;
 }
 
@@ -381,13 +419,15 @@ inline int GroupLanes(int subtype, const Plan& pl) { return subtype == 2 || pl.b
 
 inline bool SameGroup(int subtype, const Plan& a, const Plan& b) {
   return a.bits == b.bits && GroupLanes(subtype, a) == GroupLanes(subtype, b) && a.cfg.k == b.cfg.k &&
-         a.cfg.waves == b.cfg.waves;
+         a.cfg.waves == b.cfg.waves && a.cfg.split == b.cfg.split;
 }
 
 // One persistent launch of a pass: the jobs [a, b) of the pass that share a
 // kernel, with its launch shape and parameters.
 struct LaunchGroup {
   size_t a, b;
+  uint32_t items;       // queue items (the tiles of its split jobs count each)
+  uint64_t item_at;     // its first item in the pass's item table
   KernelFn fn;
   int threads;
   uint32_t lds;   // query-profile bytes (4 bases x K/2 words a lane), or more
@@ -407,6 +447,14 @@ constexpr uint64_t kBatchSegment = 4ull << 30;      // batched passes
 constexpr uint64_t kPerCallSegment = 64ull << 20;   // one alignment per call
 int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::vector<uint64_t>* addr,
                   uint64_t min_segment);
+
+// Split jobs (defined in poa_hip.hip): the engine's hand-off words grown to
+// at least `words` (zeroed when allocated; no launch of the engine may be
+// running), and the next launch's tag.
+bool ReserveXfer(spoa_hip_engine* E, uint64_t words);
+uint32_t NextEpoch(spoa_hip_engine* E);
+inline constexpr const char* kSplitTimeoutMsg =
+    "[spoa::hip] the row tiles of a split alignment timed out waiting for each other";
 
 // Launch groups of plans[lo, hi) (sorted by group, largest V * n first in
 // one); defined in poa_hip.hip.

@@ -145,6 +145,23 @@ int PlaceMatrices(spoa_hip_engine* E, const std::vector<uint64_t>& need, std::ve
   return SPOA_HIP_OK;
 }
 
+// The hand-off words of a pass's split jobs: grown (and zeroed, so that no
+// word carries a tag before some launch writes it) while no launch of the
+// engine uses them.
+bool ReserveXfer(spoa_hip_engine* E, uint64_t words) {
+  const uint64_t bytes = words * 8;
+  if (E->d_xfer.size >= bytes) return true;
+  if (!E->d_xfer.Reserve(bytes, E->msg->no_memory)) return false;
+  return HipOk(hipMemsetAsync(E->d_xfer.p, 0, E->d_xfer.size, E->copy), "hipMemsetAsync") &&
+         HipOk(hipStreamSynchronize(E->copy), "hipStreamSynchronize");
+}
+
+// The tag of a launch's hand-off words: never 0 (the zeroed words).
+uint32_t NextEpoch(spoa_hip_engine* E) {
+  if (++E->epoch == 0) E->epoch = 1;
+  return E->epoch;
+}
+
 // Launch groups of a pass: plans[lo, hi) sorted by launch group, the largest
 // V * n first inside a group.  Per group: its kernel, workgroup size, LDS,
 // grid (workgroups that fit the device at once, at most one a job) and the
@@ -154,12 +171,19 @@ int BuildGroups(spoa_hip_engine* E, const std::vector<Plan>& plans, size_t lo, s
   const spoa_hip_params& P = E->prm;
   std::vector<LaunchGroup>& groups = *out;
   groups.clear();
+  uint64_t item_at = 0;
   for (size_t x = lo; x < hi;) {
     size_t y = x;
     while (y < hi && SameGroup(E->prm.subtype, plans[y], plans[x])) ++y;
     LaunchGroup gp;
     gp.a = x - lo;
     gp.b = y - lo;
+    uint64_t items = 0;
+    for (size_t z = x; z < y; ++z) items += plans[z].tiles;
+    if (items > 0xffffffffull) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] too many row tiles in one launch");
+    gp.items = static_cast<uint32_t>(items);
+    gp.item_at = item_at;
+    item_at += items;
     gp.fn = GroupKernel(E, plans[x]);
     gp.threads = 64 * plans[x].cfg.waves;
     gp.lds = 8u * static_cast<uint32_t>(plans[x].cfg.k) * static_cast<uint32_t>(gp.threads);
@@ -179,7 +203,7 @@ int BuildGroups(spoa_hip_engine* E, const std::vector<Plan>& plans, size_t lo, s
     };
     int per_cu = 0;
     if (!blocks_per_cu(gp.lds, &per_cu)) return SPOA_HIP_DEVICE_ERROR;
-    const uint32_t n = static_cast<uint32_t>(gp.b - gp.a);
+    const uint32_t n = gp.items;
     // Workgroups take the LDS of their query profile (the traceback's tile
     // lives in it), enlarged to 24 or 32 KiB only where that leaves the
     // kernel's own workgroups per CU (set by its registers) unchanged:
@@ -547,10 +571,25 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     seq_bytes += plans[x].seq_bytes;
     pairs += jobs[plans[x].job].graph.num_nodes + jobs[plans[x].job].len;
   }
+  // split jobs: their hand-off words and the pass's queue items (every
+  // tile of a split job is an item, the tiles of a job in order)
+  std::vector<uint64_t> x_at(nj, 0);
+  uint64_t xfer_words = spoa_hip_dev::kXferFirst, n_items = 0;
+  bool any_split = false;
+  for (size_t x = lo; x < hi; ++x) {
+    n_items += plans[x].tiles;
+    if (plans[x].tiles > 1) {
+      any_split = true;
+      x_at[x - lo] = xfer_words;
+      xfer_words += XferWords(jobs[plans[x].job].graph.num_nodes, plans[x].tiles);
+    }
+  }
+  if (xfer_words > 0xffffffffull) return Fail(SPOA_HIP_NO_MEMORY, E->msg->no_memory);
   const uint64_t off_jobs = 0;
   const uint64_t off_graph = RoundUp(nj * sizeof(DevJob), kAlign);
   const uint64_t off_seq = RoundUp(off_graph + graph_words * 4, kAlign);
-  const uint64_t in_bytes = RoundUp(off_seq + seq_bytes, kAlign);
+  const uint64_t off_items = RoundUp(off_seq + seq_bytes, kAlign);
+  const uint64_t in_bytes = RoundUp(off_items + (any_split ? n_items * sizeof(uint2) : 0), kAlign);
   const uint64_t off_res = 0;
   const uint64_t off_pairs = RoundUp(static_cast<uint64_t>(nj) * 16, kAlign);
   const uint64_t out_bytes = off_pairs + pairs * 8;
@@ -561,6 +600,7 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
       !E->d_counter.Reserve(256 * sizeof(uint32_t), E->msg->no_memory)) {
     return SPOA_HIP_NO_MEMORY;
   }
+  if (any_split && !ReserveXfer(E, xfer_words)) return SPOA_HIP_NO_MEMORY;
   // place every job's matrices in an arena segment (first fit in order)
   std::vector<uint64_t> addr(nj);
   {
@@ -589,6 +629,9 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
       d.stride = pl.stride;
       d.pair_off = static_cast<uint32_t>(p_at[x]);
       d.cap = j.graph.num_nodes + j.len;
+      d.split = pl.tiles > 1 ? pl.tiles : 0u;
+      d.xfer_off = static_cast<uint32_t>(x_at[x]);
+      d.pad0 = d.pad1 = 0;
       PackJob(P, j, pl, E->h_in.At<uint32_t>(off_graph) + g_at[x], E->h_in.At<uint8_t>(off_seq) + s_at[x],
               &d.nslots, &d.slot_chars, &job_r[x]);
     }
@@ -615,6 +658,20 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     };
     if (!copy(off_jobs + a * sizeof(DevJob), (b - a) * sizeof(DevJob)) || !copy(off_graph + g0, g1 - g0) ||
         !copy(off_seq + s0, s1 - s0)) {
+      return SPOA_HIP_DEVICE_ERROR;
+    }
+  }
+  if (any_split) {
+    // the item table: (job index inside its launch group, tile)
+    uint2* it = E->h_in.At<uint2>(off_items);
+    size_t g0 = lo;
+    for (size_t x = lo; x < hi; ++x) {
+      if (x > lo && !SameGroup(P.subtype, plans[x], plans[x - 1])) g0 = x;
+      for (uint32_t t = 0; t < plans[x].tiles; ++t) *it++ = make_uint2(static_cast<uint32_t>(x - g0), t);
+    }
+    if (nslices > 1 && !HipOk(hipMemcpyAsync(static_cast<uint8_t*>(E->d_in.p) + off_items, E->h_in.At<uint8_t>(off_items),
+                                             n_items * sizeof(uint2), hipMemcpyHostToDevice, E->copy),
+                              "copy in")) {
       return SPOA_HIP_DEVICE_ERROR;
     }
   }
@@ -652,6 +709,9 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
   uint8_t* din = static_cast<uint8_t*>(E->d_in.p);
   uint8_t* dout = static_cast<uint8_t*>(E->d_out.p);
   uint32_t* counters = static_cast<uint32_t*>(E->d_counter.p);
+  // every launch tags its hand-off words with an epoch of its own
+  std::vector<uint32_t> epochs(groups.size());
+  for (auto& ep : epochs) ep = NextEpoch(E);
   hipStream_t st = E->copy;
   hipStream_t cs = E->compute;
   auto args_of = [&](size_t gi) {
@@ -668,6 +728,13 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
     args.lds_bytes = gp.lds;
     args.prio_jobs = gp.prio;
     args.phases = E->phases;
+    if (any_split) {
+      args.items = reinterpret_cast<const uint2*>(din + off_items) + gp.item_at;
+      args.n_items = gp.items;
+    }
+    args.epoch = epochs[gi];
+    args.wait_ticks = E->wait_ticks;
+    args.xfer = static_cast<unsigned long long*>(E->d_xfer.p);
     return args;
   };
   E->pass_width.clear();
@@ -833,9 +900,10 @@ int RunPassFinish(spoa_hip_engine* E) {
       j.has_score = r4[2];
       j.score = r4[1];
       const uint32_t np = static_cast<uint32_t>(r4[0]);
-      int status = r4[3] == spoa_hip_dev::kDevCapacity
-                       ? SPOA_HIP_CAPACITY
-                       : (r4[3] == spoa_hip_dev::kDevInconsistent ? SPOA_HIP_INCONSISTENT : SPOA_HIP_OK);
+      int status = r4[3] == spoa_hip_dev::kDevCapacity       ? SPOA_HIP_CAPACITY
+                   : r4[3] == spoa_hip_dev::kDevInconsistent ? SPOA_HIP_INCONSISTENT
+                   : r4[3] == spoa_hip_dev::kDevTimeout      ? SPOA_HIP_DEVICE_ERROR
+                                                             : SPOA_HIP_OK;
       if (status == SPOA_HIP_OK && np > j.cap) status = SPOA_HIP_CAPACITY;
       if (status == SPOA_HIP_OK) {
         const int32_t* src = allp + 2ull * pd.pair_at[x - lo];
@@ -857,8 +925,9 @@ int RunPassFinish(spoa_hip_engine* E) {
     const int status = jobs[plans[x].job].status;
     if (status != SPOA_HIP_OK) {
       first = status;
-      g_last_error = status == SPOA_HIP_CAPACITY ? "[spoa::hip] alignment output capacity exceeded"
-                                                 : "[spoa::hip] traceback found no consistent move";
+      g_last_error = status == SPOA_HIP_CAPACITY       ? "[spoa::hip] alignment output capacity exceeded"
+                     : status == SPOA_HIP_DEVICE_ERROR ? kSplitTimeoutMsg
+                                                       : "[spoa::hip] traceback found no consistent move";
     }
   }
   E->stats.unpack_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wait1).count();
@@ -935,6 +1004,11 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     const int L = sh.lanes;
     pl.W = (j.len + L - 1) / L * L;
     pl.stride = static_cast<uint32_t>(RoundUp(pl.W, static_cast<uint64_t>(pl.cfg.k)));
+    pl.tiles = JobTiles(pl.stride, pl.cfg);
+    if (pl.tiles > kMaxSplitTiles) {  // such long rows run their tiles in turn
+      pl.cfg.split = 0;
+      pl.tiles = 1;
+    }
     pl.mat_bytes = JobBytes(P, V, pl.stride, sh.bits, pl.cfg);
     const uint32_t E_in = j.graph.pred_off[V] - j.graph.pred_off[0];
     pl.graph_words = RoundUp(9ull * V + 4 + E_in, 4);
@@ -955,6 +1029,7 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
     if (la != lb) return la > lb;
     if (a.cfg.k != b.cfg.k) return a.cfg.k < b.cfg.k;
     if (a.cfg.waves != b.cfg.waves) return a.cfg.waves < b.cfg.waves;
+    if (a.cfg.split != b.cfg.split) return a.cfg.split < b.cfg.split;
     const uint64_t wa = static_cast<uint64_t>(jobs[a.job].graph.num_nodes) * jobs[a.job].len;
     const uint64_t wb = static_cast<uint64_t>(jobs[b.job].graph.num_nodes) * jobs[b.job].len;
     return wa > wb;
@@ -1281,18 +1356,25 @@ int spoa_hip_last_stats(spoa_hip_engine* E, spoa_hip_stats* out) {
   return SPOA_HIP_OK;
 }
 
-int spoa_hip_config(spoa_hip_engine* E, uint32_t len, int32_t width_bits, int32_t batch, int32_t* k, int32_t* waves) {
-  if (!E || !k || !waves || (width_bits != 16 && width_bits != 32)) {
+int spoa_hip_config_split(spoa_hip_engine* E, uint32_t len, int32_t width_bits, int32_t batch, int32_t* k,
+                          int32_t* waves, int32_t* split) {
+  if (!E || !k || !waves || !split || (width_bits != 16 && width_bits != 32)) {
     return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] invalid argument");
   }
   const Cfg c = ChooseCfg(E, len, width_bits, batch != 0);
   *k = c.k;
   *waves = c.waves;
+  *split = c.split;
   return SPOA_HIP_OK;
 }
 
-int spoa_hip_set_config(spoa_hip_engine* E, int32_t width_bits, int32_t batch, uint32_t max_width, int32_t k,
-                        int32_t waves) {
+int spoa_hip_config(spoa_hip_engine* E, uint32_t len, int32_t width_bits, int32_t batch, int32_t* k, int32_t* waves) {
+  int32_t split = 0;
+  return spoa_hip_config_split(E, len, width_bits, batch, k, waves, &split);
+}
+
+int spoa_hip_set_config_split(spoa_hip_engine* E, int32_t width_bits, int32_t batch, uint32_t max_width, int32_t k,
+                              int32_t waves, int32_t split) {
   if (!E || (width_bits != 16 && width_bits != 32)) {
     return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] invalid argument");
   }
@@ -1308,7 +1390,19 @@ int spoa_hip_set_config(spoa_hip_engine* E, int32_t width_bits, int32_t batch, u
   Cfg c;
   c.k = k;
   c.waves = waves;
+  c.split = split != 0 ? 1 : 0;
   table[max_width] = c;
+  return SPOA_HIP_OK;
+}
+
+int spoa_hip_set_config(spoa_hip_engine* E, int32_t width_bits, int32_t batch, uint32_t max_width, int32_t k,
+                        int32_t waves) {
+  return spoa_hip_set_config_split(E, width_bits, batch, max_width, k, waves, 0);
+}
+
+int spoa_hip_set_wait_limit(spoa_hip_engine* E, uint32_t ticks) {
+  if (!E) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] null engine");
+  E->wait_ticks = ticks;
   return SPOA_HIP_OK;
 }
 

@@ -141,6 +141,49 @@ __device__ __forceinline__ uint32_t Uniform(uint32_t v) {
   return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
 }
 
+// ------------------------------------------------- split-job hand-off --
+// The row tiles of a split job run on different workgroups (possibly on
+// different XCDs, whose L2 caches are not coherent with each other).  They
+// exchange 64-bit words: the launch's tag in the high half, a value in the
+// low half, stored and loaded with device-scope relaxed atomics (vector
+// accesses that go to the device's coherence point).  A word is valid once
+// it carries the launch's tag, so no fence orders the per-row hand-off;
+// only the traceback, which reads every tile's rows, pairs a release fence
+// of each tile with an acquire fence of the tile that walks it.
+__device__ __forceinline__ void XferPut(unsigned long long* p, uint32_t tag, int32_t v) {
+  __hip_atomic_store(p, (static_cast<unsigned long long>(tag) << 32) | static_cast<uint32_t>(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long XferGet(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool XferTagged(unsigned long long w, uint32_t tag) {
+  return static_cast<uint32_t>(w >> 32) == tag;
+}
+
+// Waits (the whole wave) until the word that each lane l < nw holds in w --
+// loaded from base[l] -- carries the launch's tag, reloading it after a
+// short sleep.  Gives up after wait_ticks of the 100 MHz clock, raising the
+// launch's abort word xfer[0], or at once when that word is already raised.
+// True when every word is valid.
+__device__ __forceinline__ bool XferWait(unsigned long long* xfer, uint32_t tag, uint32_t wait_ticks,
+                                         unsigned long long* base, int nw, int lane, unsigned long long& w) {
+  auto pending = [&]() { return __builtin_amdgcn_ballot_w64(lane < nw && !XferTagged(w, tag)) != 0; };
+  if (!pending()) return true;
+  const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+  for (;;) {
+    const unsigned long long ab = XferGet(xfer);
+    if (__builtin_amdgcn_ballot_w64(XferTagged(ab, tag)) != 0) return false;
+    __builtin_amdgcn_s_sleep(1);
+    if (lane < nw) w = XferGet(base + lane);
+    if (!pending()) return true;
+    if (__builtin_amdgcn_s_memrealtime() - t0 > wait_ticks) {
+      if (lane == 0) XferPut(xfer, tag, 1);
+      return false;
+    }
+  }
+}
+
 // The K read bases at columns c0.. (4 per word) and how many of them lie
 // inside the read; the sequence block is padded to the row stride.
 template <int K>
@@ -895,12 +938,20 @@ PoaAlignKernel(const KArgs a) {
     }
   }
 
+  const uint32_t n_items = a.items ? a.n_items : a.n_jobs;
   for (;;) {
     if (tid == 0) s_job = atomicAdd(a.queue, 1u);
     __syncthreads();
-    const uint32_t jx = Uniform(s_job);
+    const uint32_t ix = Uniform(s_job);
     __syncthreads();
-    if (jx >= a.n_jobs) break;
+    if (ix >= n_items) break;
+    // the job and, for a split job, the row tile this workgroup aligns
+    uint32_t jx = ix, tile = 0;
+    if (a.items) {
+      const uint2 it = a.items[ix];
+      jx = Uniform(it.x);
+      tile = Uniform(it.y);
+    }
     // The launch lasts as long as its largest jobs (the queue hands them
     // out first): their waves take the SIMDs' issue slots before those of
     // the smaller jobs beside them, which have time to spare.
@@ -925,6 +976,8 @@ PoaAlignKernel(const KArgs a) {
     J.cap = Uniform(Jv.cap);
     J.nslots = Uniform(Jv.nslots);
     J.slot_chars = Uniform(Jv.slot_chars);
+    J.split = Uniform(Jv.split);
+    J.xfer_off = Uniform(Jv.xfer_off);
     const uint32_t V = J.V, n = J.n, W = J.W, stride = J.stride;
     const GraphView gv = MakeView(a.graphs + J.graph_off, V);
     const uint8_t* __restrict__ seq = a.seqs + J.seq_off;
@@ -934,7 +987,23 @@ PoaAlignKernel(const KArgs a) {
     const uint32_t mbytes = stride * S;       // one matrix of a row
     const uint32_t tile_w = static_cast<uint32_t>(nthreads) * K;
     const uint32_t ntiles = (stride + tile_w - 1) / tile_w;
-    const bool single = ntiles == 1;  // a row is one tile: the previous row stays in registers
+    // The tiles of every row this workgroup aligns: all of them in turn, or
+    // -- a split job, whose ntiles tiles run on as many workgroups -- one.
+    // A split tile t > 0 receives, per row, the H of tile t - 1's last
+    // column and the carry entering its first column through hand-off words
+    // (xin); a tile before the last sends them (xout).
+    const bool split = J.split > 1;
+    const uint32_t t_begin = split ? tile : 0u, t_end = split ? tile + 1 : ntiles;
+    const bool single = t_end - t_begin == 1;  // one tile a row: the previous row stays in registers
+    unsigned long long* xin = nullptr;
+    unsigned long long* xout = nullptr;
+    if (split) {
+      unsigned long long* xj = a.xfer + J.xfer_off;
+      if (tile > 0) xin = xj + 3ull * (tile - 1) * (V + 1);
+      if (tile + 1 < ntiles) xout = xj + 3ull * tile * (V + 1);
+    }
+    constexpr int kXw = SUB == 2 ? 3 : 2;  // hand-off words a row: last H, carry (two for convex)
+    bool aborted = false;
     // A row is nseg = ntiles * nwaves wave segments.  The first lane of
     // segment sg >= 1 needs, for the diagonal, the H value left of it in
     // every predecessor row: it keeps them in bnd[sg][row] (its own stores
@@ -954,7 +1023,8 @@ PoaAlignKernel(const KArgs a) {
       if (SUB == 1) return g + ci * e;
       return max(g + ci * e, q + ci * cc);
     };
-    for (uint32_t col = tid; col < stride; col += nthreads) {
+    const uint32_t col_end = min(stride, t_end * tile_w);
+    for (uint32_t col = t_begin * tile_w + tid; col < col_end; col += nthreads) {
       const int32_t ci = static_cast<int32_t>(col);
       mat[col] = static_cast<ST>(row0_h(ci));
       if (SUB >= 1) {
@@ -972,7 +1042,8 @@ PoaAlignKernel(const KArgs a) {
     // lane's query profile in LDS (written and read by this lane only)
     uint32_t scp[K / 4];
     int32_t nvalid;
-    LoadChars<K>(seq, static_cast<uint32_t>(tid) * K, n, static_cast<uint32_t>(tid) * K < stride, scp, nvalid);
+    const uint32_t lane_c0 = t_begin * tile_w + static_cast<uint32_t>(tid) * K;  // first tile's columns
+    LoadChars<K>(seq, lane_c0, n, lane_c0 < stride, scp, nvalid);
     for (uint32_t sl = 0; sl < nslots; ++sl) {
       uint32_t pw[PW];
       ProfileWords<K>(scp, nvalid, (J.slot_chars >> (8 * sl)) & 0xffu, m, mis, pad, pw);
@@ -985,7 +1056,7 @@ PoaAlignKernel(const KArgs a) {
     if (single) {
 #pragma unroll
       for (int i = 0; i < KW; ++i) {
-        const int32_t c = tid * K + i * CPW;
+        const int32_t c = static_cast<int32_t>(lane_c0) + i * CPW;
         if constexpr (CPW == 2) {
           cH[i] = (static_cast<uint32_t>(row0_h(c)) & 0xffffu) | (static_cast<uint32_t>(row0_h(c + 1)) << 16);
         } else {
@@ -1015,8 +1086,9 @@ PoaAlignKernel(const KArgs a) {
     int32_t pf_hm1 = 0;  // with the prefetch: the predecessor's H left of the segment (first lane)
     // one-tile rows: H of the previous row left of this wave's segment
     int32_t bprev = 0;
-    if (single && wave > 0) {
-      bprev = static_cast<int32_t>(static_cast<ST>(row0_h(wave * 64 * K - 1)));
+    const uint32_t seg_c0 = t_begin * tile_w + static_cast<uint32_t>(wave) * 64u * K;
+    if (single && seg_c0 > 0) {
+      bprev = static_cast<int32_t>(static_cast<ST>(row0_h(static_cast<int32_t>(seg_c0) - 1)));
     }
 
     for (uint32_t r = 1; r <= V; ++r) {
@@ -1039,7 +1111,13 @@ PoaAlignKernel(const KArgs a) {
       int32_t y0 = fch_r + g;
       int32_t y1 = fch_r + q;
 
-      for (uint32_t t = 0; t < ntiles; ++t) {
+      // a split tile's hand-off words of this row, loaded now and checked
+      // before the scan (the tile before is usually rows ahead)
+      unsigned long long xw = 0ull;
+      if (xin != nullptr && lane < kXw) xw = XferGet(xin + 3ull * r + lane);
+      int32_t hl = 0;  // tile t - 1's H at its last column, this row
+
+      for (uint32_t t = t_begin; t < t_end; ++t) {
         const uint32_t c0 = t * tile_w + static_cast<uint32_t>(tid) * K;
         const bool active = c0 < stride;
         const uint32_t off = c0 * S;  // byte offset of the lane's columns in a matrix
@@ -1160,14 +1238,14 @@ PoaAlignKernel(const KArgs a) {
             {
               bool want = false;
               uint32_t q = 0, qc0 = 0;
-              if (t + 1 < ntiles) {
+              if (t + 1 < t_end) {
                 want = true;
                 q = p0;
                 qc0 = c0 + tile_w;
               } else if (r < V && !(single && nxt.y == r)) {
                 want = true;
                 q = nxt.y;
-                qc0 = static_cast<uint32_t>(tid) * K;
+                qc0 = lane_c0;
               }
               if (want) {
                 const __amdgpu_buffer_rsrc_t rq = RowRsrc(row_ptr(q), rbytes);
@@ -1177,7 +1255,7 @@ PoaAlignKernel(const KArgs a) {
                 if (SUB >= 1) BufLoad<KW>(rq, qa ? qoff + mbytes : rbytes, pfF);
                 if (SUB == 2) BufLoad<KW>(rq, qa ? qoff + 3 * mbytes : rbytes, pfO);
                 if (lane == 0 && qc0 > 0 && q > 0) {
-                  const uint32_t qt = t + 1 < ntiles ? t + 1 : 0u;
+                  const uint32_t qt = t + 1 < t_end ? t + 1 : t_begin;
                   pf_hm1 = bnd[static_cast<uint64_t>(qt * static_cast<uint32_t>(nwaves) + wave) * (V + 1) + q];
                 }
                 pf_valid = true;
@@ -1312,6 +1390,18 @@ PoaAlignKernel(const KArgs a) {
             b0 = max(hf + g, xe + e);
             b1 = max(hf + q, xq + cc);
           }
+        }
+
+        // a split tile after the first: the carry entering its first column
+        // and tile t - 1's H at its last column come from the hand-off words
+        // (after a failed wait they are not awaited again: the job reports
+        // the time-out)
+        if (xin != nullptr) {
+          if (!aborted && !XferWait(a.xfer, a.epoch, a.wait_ticks, xin + 3ull * r, kXw, lane, xw)) aborted = true;
+          const int32_t lo = static_cast<int32_t>(static_cast<uint32_t>(xw));
+          hl = __builtin_amdgcn_readlane(lo, 0);
+          y0 = __builtin_amdgcn_readlane(lo, 1);
+          if constexpr (SUB == 2) y1 = __builtin_amdgcn_readlane(lo, 2);
         }
 
         const unsigned long long tm3 = SPOA_PH_NOW();
@@ -1562,6 +1652,18 @@ PoaAlignKernel(const KArgs a) {
 #pragma unroll
           for (int i = 0; i < KW; ++i) cH[i] = hw[i];
         }
+        // a split tile before the last: hand this row's last H and the carry
+        // out of the tile to the next tile (its columns are all inside the
+        // row, so the last lane of the last wave holds them)
+        if (xout != nullptr && wave == nwaves - 1) {
+          const int32_t last = LastCol<ST, KW>(hw);
+          if (lane == 63) {
+            unsigned long long* xo = xout + 3ull * r;
+            XferPut(xo, a.epoch, last);
+            XferPut(xo + 1, a.epoch, tile_out0);
+            if constexpr (SUB == 2) XferPut(xo + 2, a.epoch, tile_out1);
+          }
+        }
 
         const unsigned long long tm5 = SPOA_PH_NOW();
         // end-cell candidates (:903-928): SW all rows, OV sink rows (row
@@ -1632,16 +1734,24 @@ PoaAlignKernel(const KArgs a) {
                 got = s_bnd[wave - 1];
                 to_sg = t * static_cast<uint32_t>(nwaves) + wave;
                 have = true;
-              } else if (t + 1 < ntiles) {
+              } else if (t + 1 < t_end) {
                 got = s_bnd[nwaves - 1];
                 to_sg = (t + 1) * static_cast<uint32_t>(nwaves);
+                have = true;
+              } else if (xin != nullptr) {
+                got = hl;
+                to_sg = t * static_cast<uint32_t>(nwaves);
                 have = true;
               }
             }
           } else {
             got = __builtin_amdgcn_readlane(last, 63);
-            if (lane == 0 && t + 1 < ntiles) {
+            if (lane == 0 && t + 1 < t_end) {
               to_sg = t + 1;
+              have = true;
+            } else if (lane == 0 && xin != nullptr) {
+              got = hl;
+              to_sg = t;
               have = true;
             }
           }
@@ -1699,15 +1809,77 @@ PoaAlignKernel(const KArgs a) {
         max_i = rr;
       }
     }
-    const bool found = max_i != 0x7fffffff;
     __syncthreads();
+
+    // A split job: a tile before the last publishes its candidate once its
+    // rows are visible device-wide (the barrier above has waited for every
+    // wave's stores) and is done; the last tile -- it finishes after all
+    // the others, whose last rows it waited for -- takes the best of all
+    // the candidates (the first row on ties, as across waves), acquires the
+    // other tiles' rows and walks the traceback.
+    bool last_tile = true;
+    if (split) {
+      unsigned long long* xc = a.xfer + J.xfer_off + 3ull * (ntiles - 1) * (V + 1);
+      if (tile + 1 < ntiles) {
+        last_tile = false;
+        if (tid == 0) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          XferPut(xc + 2 * tile, a.epoch, max_score);
+          XferPut(xc + 2 * tile + 1, a.epoch, max_i);
+        }
+      } else {
+        if (wave == 0) {
+          for (uint32_t u0 = 0; u0 + 1 < ntiles; u0 += 32) {
+            const uint32_t nu = min(32u, ntiles - 1 - u0);
+            const int nw = static_cast<int>(2 * nu);
+            unsigned long long w = 0ull;
+            if (lane < nw) w = XferGet(xc + 2ull * u0 + lane);
+            if (!aborted && !XferWait(a.xfer, a.epoch, a.wait_ticks, xc + 2ull * u0, nw, lane, w)) aborted = true;
+            const int32_t lo = static_cast<int32_t>(static_cast<uint32_t>(w));
+            int32_t v = __shfl(lo, (2 * lane) & 63, 64), rr = __shfl(lo, (2 * lane + 1) & 63, 64);
+            if (static_cast<uint32_t>(lane) >= nu) {
+              v = kMinI;
+              rr = 0x7fffffff;
+            }
+#pragma unroll
+            for (int st = 16; st >= 1; st >>= 1) {
+              const int32_t ov2 = __shfl_xor(v, st, 64);
+              const int32_t or2 = __shfl_xor(rr, st, 64);
+              if (ov2 > v || (ov2 == v && or2 < rr)) {
+                v = ov2;
+                rr = or2;
+              }
+            }
+            v = __builtin_amdgcn_readfirstlane(v);
+            rr = __builtin_amdgcn_readfirstlane(rr);
+            if (v > max_score || (v == max_score && rr < max_i)) {
+              max_score = v;
+              max_i = rr;
+            }
+          }
+          if (__builtin_amdgcn_ballot_w64(XferTagged(XferGet(a.xfer), a.epoch)) != 0) aborted = true;
+          if (lane == 0) {
+            s_red[0][0] = max_score;
+            s_red[0][1] = max_i;
+            s_red[0][2] = aborted ? 1 : 0;
+          }
+        }
+        __syncthreads();
+        max_score = s_red[0][0];
+        max_i = s_red[0][1];
+        aborted = s_red[0][2] != 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __syncthreads();
+      }
+    }
+    const bool found = max_i != 0x7fffffff;
 
     // max_j (:937-953): NW the last column; SW/OV the first column of row
     // max_i (padding included; the scalar engine: the read's columns)
     // holding the best value, -1 if none
     const uint64_t rstride = static_cast<uint64_t>(stride) * NM;
     int32_t max_j = static_cast<int32_t>(n) - 1;
-    if (found && type != 1) {
+    if (last_tile && !aborted && found && type != 1) {
       const ST* __restrict__ hrow = mat + static_cast<uint64_t>(max_i) * rstride;
       const uint32_t cols = sisd ? n : W;
       int32_t first = 0x7fffffff;
@@ -1731,9 +1903,16 @@ PoaAlignKernel(const KArgs a) {
     SPOA_PH_ADD(kPhReduce, tr1 - tr0);
     (void)tr0, (void)tr1;
     // ---- traceback by wave 0 (the query profile's LDS is free now)
-    if (wave == 0) {
+    if (last_tile && wave == 0) {
       int32_t* res = a.results + 4 * jx;
-      if (!found) {
+      if (aborted) {
+        if (lane == 0) {
+          res[0] = 0;
+          res[1] = 0;
+          res[2] = 0;
+          res[3] = kDevTimeout;
+        }
+      } else if (!found) {
         if (lane == 0) {
           res[0] = 0;
           res[1] = 0;

@@ -36,6 +36,15 @@ struct DevJob {
   uint32_t cap;        // pair capacity (V + n suffices)
   uint32_t nslots;     // distinct row bases with an LDS profile (0: formed per row)
   uint32_t slot_chars; // the base of each profile slot, a byte each
+  // Split jobs: the number of row tiles, each aligned by its own workgroup
+  // (0 or 1: one workgroup runs whole rows), and the job's hand-off words
+  // in KArgs::xfer (64-bit words): per boundary b = 1 .. split - 1 and row
+  // r, words 3 * ((b - 1) * (V + 1) + r) .. + 2 hold the H of tile b - 1's
+  // last column and the carry (two words) entering tile b; after them two
+  // words a tile with its end-cell candidate (score, row).
+  uint32_t split;
+  uint32_t xfer_off;
+  uint32_t pad0, pad1;
 };
 
 // Graph block of a job (32-bit words from graph_off, 16-byte aligned):
@@ -95,8 +104,20 @@ struct KArgs {
   uint32_t lds_bytes;        // dynamic LDS of the launch (at least the query profile's)
   uint32_t prio_jobs;        // the first prio_jobs jobs of the queue run at raised wave priority
   unsigned long long* phases;  // row-phase cycle counters (builds with SPOA_ROW_PHASES; else unused)
+  // Queue items of a launch with split jobs: per item the job (index into
+  // jobs) and the row tile it aligns, the tiles of a job consecutive and in
+  // order (a tile only waits for items taken from the queue before it).
+  // Null: item i is job i, whole rows.
+  const uint2* items;
+  uint32_t n_items;
+  uint32_t epoch;       // tag of this launch's hand-off words (never 0)
+  uint32_t wait_ticks;  // longest wait for another tile (100 MHz clock ticks)
+  unsigned long long* xfer;  // hand-off words; word 0: the launch's abort word
 };
 
-enum DevStatus : int32_t { kDevOk = 0, kDevCapacity = 6, kDevInconsistent = 7 };
+enum DevStatus : int32_t { kDevOk = 0, kDevCapacity = 6, kDevInconsistent = 7, kDevTimeout = 8 };
+
+// Hand-off words of split jobs start here (words 0-1: the abort word).
+constexpr uint32_t kXferFirst = 2;
 
 }  // namespace spoa_hip_dev

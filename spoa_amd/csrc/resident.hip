@@ -567,6 +567,11 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
       pl.cfg = ChooseCfg(E, cfg_len[sh.cfgw == 32], sh.cfgw, true);
       pl.W = (len + sh.lanes - 1) / sh.lanes * sh.lanes;
       pl.stride = static_cast<uint32_t>(RoundUp(pl.W, static_cast<uint64_t>(pl.cfg.k)));
+      pl.tiles = JobTiles(pl.stride, pl.cfg);
+      if (pl.tiles > kMaxSplitTiles) {  // such long rows run their tiles in turn
+        pl.cfg.split = 0;
+        pl.tiles = 1;
+      }
       pl.mat_bytes = JobBytes(P, V, pl.stride, sh.bits, pl.cfg);
       pl.graph_words = 0;  // the window's block is resident
       pl.seq_bytes = 0;
@@ -582,6 +587,7 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
     if (la != lb) return la > lb;
     if (a.cfg.k != b.cfg.k) return a.cfg.k < b.cfg.k;
     if (a.cfg.waves != b.cfg.waves) return a.cfg.waves < b.cfg.waves;
+    if (a.cfg.split != b.cfg.split) return a.cfg.split < b.cfg.split;
     return a.work > b.work;
   });
   const size_t nj = plans.size();
@@ -594,7 +600,15 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
   if (pair_at[nj] > 0xffffffffull) return WinFail(X, SPOA_HIP_NO_MEMORY, E->msg->no_memory);
   const auto t_alloc0 = std::chrono::steady_clock::now();
   const uint64_t off_items = RoundUp(nj * sizeof(DevJob), kAlign);
-  const uint64_t table_bytes = RoundUp(off_items + X->live.size() * sizeof(MergeItem), kAlign) + kAlign;
+  // queue items of split jobs (every tile an item), after the merge items
+  uint64_t n_tiles = 0;
+  bool any_split = false;
+  for (size_t x = 0; x < nj; ++x) {
+    n_tiles += plans[x].tiles;
+    any_split |= plans[x].tiles > 1;
+  }
+  const uint64_t off_tiles = RoundUp(off_items + X->live.size() * sizeof(MergeItem), kAlign);
+  const uint64_t table_bytes = RoundUp(off_tiles + (any_split ? n_tiles * sizeof(uint2) : 0), kAlign) + kAlign;
   if (!X->h_table.Reserve(table_bytes) || !X->d_table.Reserve(table_bytes, E->msg->no_memory) ||
       !X->d_res.Reserve(std::max<uint64_t>(nj, 1) * 16, E->msg->no_memory) ||
       !X->d_pairs.Reserve(std::max<uint64_t>(pair_at[nj], 1) * 8, E->msg->no_memory)) {
@@ -646,6 +660,32 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
       E->arena_bytes = want;
     }
   }
+  // hand-off words of split jobs: the passes run one after another on the
+  // stream, so each pass's words start over (and every launch has its own
+  // epoch); sized for the largest pass before anything launches
+  std::vector<uint64_t> x_at(nj, 0);
+  if (any_split) {
+    uint64_t most = 0;
+    for (const auto& ps : passes) {
+      uint64_t w = spoa_hip_dev::kXferFirst;
+      for (size_t x = ps.first; x < ps.second; ++x) {
+        if (plans[x].tiles > 1) {
+          x_at[x] = w;
+          w += XferWords(X->state[refs[plans[x].job].window].V, plans[x].tiles);
+        }
+      }
+      most = std::max(most, w);
+    }
+    if (most > 0xffffffffull || !ReserveXfer(E, most)) return WinFail(X, SPOA_HIP_NO_MEMORY, E->msg->no_memory);
+    uint2* it = X->h_table.At<uint2>(off_tiles);
+    for (const auto& ps : passes) {
+      size_t g0 = ps.first;
+      for (size_t x = ps.first; x < ps.second; ++x) {
+        if (x > ps.first && !SameGroup(P.subtype, plans[x], plans[x - 1])) g0 = x;
+        for (uint32_t t = 0; t < plans[x].tiles; ++t) *it++ = make_uint2(static_cast<uint32_t>(x - g0), t);
+      }
+    }
+  }
   DevJob* dj = X->h_table.At<DevJob>(0);
   std::vector<int32_t> job_of(refs.size(), -1);  // ref -> table index
   std::vector<std::vector<uint64_t>> pass_addr(passes.size());
@@ -673,6 +713,8 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
       d.cap = d.V + d.n;
       d.nslots = X->state[w].nslots;
       d.slot_chars = X->state[w].slot_chars;
+      d.split = pl.tiles > 1 ? pl.tiles : 0u;
+      d.xfer_off = static_cast<uint32_t>(x_at[x]);
       job_of[pl.job] = static_cast<int32_t>(x);
       const uint64_t jb = static_cast<uint64_t>(pl.bits / 8) * d.n *
                           (static_cast<uint64_t>(Matrices(P.subtype)) * d.V +
@@ -704,6 +746,7 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
   // the DP, pass by pass
   X->launches = 0;
   X->launch_width.clear();
+  uint64_t pass_tiles = 0;  // items of the passes before this one
   for (size_t pi = 0; pi < passes.size(); ++pi) {
     const size_t lo = passes[pi].first, hi = passes[pi].second;
     std::vector<LaunchGroup> groups;
@@ -734,6 +777,13 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
       args.lds_bytes = gp.lds;
       args.prio_jobs = gp.prio;
       args.phases = E->phases;
+      if (any_split) {
+        args.items = reinterpret_cast<const uint2*>(dtab + off_tiles) + pass_tiles + gp.item_at;
+        args.n_items = gp.items;
+      }
+      args.epoch = NextEpoch(E);
+      args.wait_ticks = E->wait_ticks;
+      args.xfer = static_cast<unsigned long long*>(E->d_xfer.p);
       if (!HipOk(hipEventRecord(X->ev_s[X->launches], st), "event")) return SPOA_HIP_DEVICE_ERROR;
       gp.fn<<<gp.grid, gp.threads, gp.lds, st>>>(args);
       if (!HipOk(hipGetLastError(), "kernel launch") || !HipOk(hipEventRecord(X->ev_e[X->launches], st), "event")) {
@@ -742,6 +792,7 @@ int spoa_hip_windows_submit(spoa_hip_windows* X, uint32_t r) {
       X->launch_width.push_back(plans[lo + gp.a].width);
       ++X->launches;
     }
+    for (size_t x = lo; x < hi; ++x) pass_tiles += plans[x].tiles;
   }
   // the merge
   if (!X->live.empty()) {
