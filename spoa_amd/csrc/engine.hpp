@@ -262,10 +262,11 @@ inline KParams MakeParams(const spoa_hip_params& p, int width_bits, int lanes, i
   return kp;
 }
 
-// The alignment kernel of a launch group (defined with the kernels in
-// poa_hip.hip): subtype, stored row width, reference lane count, columns
-// per lane and waves.
-KernelFn PickKernel(int subtype, int bits, int lanes, int k, int waves);
+// The alignment kernel of a launch group (defined in poa_hip.hip):
+// subtype, stored row width, reference lane count, columns per lane, waves,
+// whether the launch may hold split jobs and whether it reproduces the
+// scalar engine.
+KernelFn PickKernel(int subtype, int bits, int lanes, int k, int waves, bool split, bool sisd);
 
 }  // namespace spoa_hip_host
 
@@ -408,7 +409,7 @@ inline uint64_t JobBytes(const spoa_hip_params& p, uint32_t V, uint32_t stride, 
 }
 
 inline KernelFn GroupKernel(spoa_hip_engine* E, const Plan& pl) {
-  return PickKernel(E->prm.subtype, pl.bits, pl.lanes, pl.cfg.k, pl.cfg.waves);
+  return PickKernel(E->prm.subtype, pl.bits, pl.lanes, pl.cfg.k, pl.cfg.waves, pl.cfg.split != 0, E->sisd);
 }
 
 // Launch group of a job: jobs of one group share a kernel and its

@@ -39,16 +39,20 @@ namespace spoa_hip_host {
 // carries, so linear and affine rows always run the kernel of their width's
 // own lane count; convex int16 rows with 8 lanes hold the local alignments
 // the reference scores at int32, and one-column segments are the scalar
-// engine's convex rows (their kernels are built in poa_sisd.hip).
-KernelFn PickKernel(int subtype, int bits, int lanes, int k, int waves) {
+// engine's convex rows.  split: the launch may hold split jobs; sisd: it
+// reproduces the scalar engine.  The families are built in poa_family.hip.
+KernelFn PickKernel(int subtype, int bits, int lanes, int k, int waves, bool split, bool sisd) {
   switch (subtype) {
-    case 0: return bits == 16 ? PickK<0, int16_t, 16>(k, waves) : PickK<0, int32_t, 8>(k, waves);
-    case 1: return bits == 16 ? PickK<1, int16_t, 16>(k, waves) : PickK<1, int32_t, 8>(k, waves);
+    case 0:
+      return bits == 16 ? PickFamily<kFamLinear16>(bits, lanes, k, waves, split, sisd)
+                        : PickFamily<kFamLinear32>(bits, lanes, k, waves, split, sisd);
+    case 1:
+      return bits == 16 ? PickFamily<kFamAffine16>(bits, lanes, k, waves, split, sisd)
+                        : PickFamily<kFamAffine32>(bits, lanes, k, waves, split, sisd);
     default:
-      if (lanes == 1) return PickConvexFullGotoh(bits, k, waves);
-      if (bits == 32) return PickK<2, int32_t, 8>(k, waves);
-      if (lanes == 8) return PickK<2, int16_t, 8>(k, waves);
-      return PickK<2, int16_t, 16>(k, waves);
+      if (lanes == 1) return PickFamily<kFamConvexGotoh>(bits, lanes, k, waves, split, sisd);
+      if (bits == 32) return PickFamily<kFamConvex32>(bits, lanes, k, waves, split, sisd);
+      return PickFamily<kFamConvex16>(bits, lanes, k, waves, split, sisd);
   }
 }
 
@@ -1009,6 +1013,9 @@ int RunBatch(spoa_hip_engine* E, spoa_hip_job* jobs, uint32_t n_jobs, bool batch
       pl.cfg.split = 0;
       pl.tiles = 1;
     }
+    // a one-tile row is not split: the job joins the launches whose kernel
+    // carries no hand-off code
+    if (pl.tiles == 1) pl.cfg.split = 0;
     pl.mat_bytes = JobBytes(P, V, pl.stride, sh.bits, pl.cfg);
     const uint32_t E_in = j.graph.pred_off[V] - j.graph.pred_off[0];
     pl.graph_words = RoundUp(9ull * V + 4 + E_in, 4);
@@ -1270,7 +1277,8 @@ int spoa_hip_align_noalloc(spoa_hip_engine* E, const spoa_hip_graph* graph, cons
                     " bytes of device scratch, more than the " + std::to_string(kScratchBytes >> 20) +
                     " MiB the no-allocation entry point has");
   }
-  KernelFn fn = PickKernel(E->prm.subtype, sh.bits, sh.lanes, cfg.k, cfg.waves);
+  // (its one job is never split: a workgroup runs the row tiles in turn)
+  KernelFn fn = PickKernel(E->prm.subtype, sh.bits, sh.lanes, cfg.k, cfg.waves, false, E->sisd);
   if (!fn) return Fail(SPOA_HIP_INVALID_ARGUMENT, "[spoa::hip] unsupported launch configuration");
   spoa_hip_dev::NoAllocPackArgs pa{};
   pa.row_char = graph->row_char;

@@ -235,7 +235,7 @@ struct TbTile {
   uint32_t pb0 = 0, pn = 0;
 };
 
-template <int SUB, typename ST>
+template <int SUB, typename ST, bool SISD>
 __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t* __restrict__ seq,
                           uint32_t n, const ST* __restrict__ mat, uint64_t rstride, uint32_t stride,
                           int32_t max_i, int32_t max_j, int32_t* __restrict__ out, uint32_t cap,
@@ -252,7 +252,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
   // and the first column (column -1 here) in the main loop, its convex left
   // walk goes on while either gap chain holds, and E / Q of the first
   // column are -inf (0 in row 0).
-  const bool sisd = prm.sisd != 0;
+  constexpr bool sisd = SISD;
   constexpr int32_t kNegInf = -2147483647 - 1 + 1024;
   auto U = [](int32_t v) -> int32_t { return static_cast<int32_t>(Uniform(static_cast<uint32_t>(v))); };
 
@@ -376,16 +376,18 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
         mat[static_cast<uint64_t>(r) * rstride + static_cast<uint64_t>(t) * stride + static_cast<uint32_t>(col)]);
   };
   // matrix t of row r at column col >= -1: column -1 is the first column
-  // (H, F, O from the graph block; E, Q -inf there, 0 in row 0)
+  // (H, F, O from the graph block; E, Q -inf there, 0 in row 0).  Only the
+  // scalar walk reaches column -1 and row 0 inside its main loop (the SIMD
+  // walk stops there), so the SIMD walk reads the matrices directly.
   auto cell = [&](int t, int32_t r, int32_t col) -> int32_t {
-    if (col >= 0) return at(t, r, col);
+    if (!SISD || col >= 0) return at(t, r, col);
     if (t == 0) return U(gv.fcH[r]);
     if (t == 1) return U(gv.fcF[r]);
     if (t == 3) return U(gv.fcO[r]);
     return r == 0 ? 0 : kNegInf;
   };
   auto cell_v = [&](int t, int32_t r, int32_t col) -> int32_t {
-    if (col >= 0) return at_v(t, r, col);
+    if (!SISD || col >= 0) return at_v(t, r, col);
     if (t == 0) return gv.fcH[r];
     if (t == 1) return gv.fcF[r];
     if (t == 3) return gv.fcO[r];
@@ -486,13 +488,15 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     const int32_t h = cell(0, i, j);
     if (type == 0 && h == 0) break;
     // row 0 has no predecessors (the scalar walk along it moves left only)
-    const uint32_t pb = i > 0 ? pred_begin(i) : 0u, pe = i > 0 ? pred_end(i) : 0u;
+    const bool row0 = SISD && i == 0;
+    const uint32_t pb = row0 ? 0u : pred_begin(i), pe = row0 ? 0u : pred_end(i);
     const bool source = pb == pe;
-    const uint32_t npr = i == 0 ? 0u : (source ? 1u : pe - pb);
+    const uint32_t npr = row0 ? 0u : (source ? 1u : pe - pb);
     bool found = false, ext_left = false, ext_up = false;
     // diagonal (:1053-1063); none from the first column
-    const int32_t pr = j >= 0 && i > 0 ? prof(i, j) : 0;
-    for (uint32_t k0 = 0; k0 < npr && !found && j >= 0; k0 += 64) {
+    const bool col_m1 = SISD && j < 0;
+    const int32_t pr = !col_m1 && !row0 ? prof(i, j) : 0;
+    for (uint32_t k0 = 0; k0 < npr && !found && !col_m1; k0 += 64) {
       const uint32_t k = k0 + static_cast<uint32_t>(lane);
       const bool act = k < npr;
       const int32_t p = act && !source ? pred_v(pb + k) : 0;
@@ -539,7 +543,7 @@ __device__ void Traceback(const KParams& prm, const GraphView& gv, const uint8_t
     // left (:1076-1083, :1459-1468, :1977-1990); column 0 reads the first
     // column (the SIMD engine's E / Q there are its H, the scalar engine's
     // -inf); none from the first column itself
-    if (!found && j >= 0) {
+    if (!found && !col_m1) {
       const int32_t hl = j > 0 ? at(0, i, j - 1) : U(gv.fcH[i]);
       bool hit;
       if (SUB == 0) {
@@ -888,7 +892,13 @@ struct WavesPerSimd {
 // [0, m * n] and every E, F, O, Q is at least its opening penalty above a
 // non-negative H (the host admits a job only when m * (n + 8) leaves int16
 // headroom).
-template <int SUB, typename ST, int K, int MAXT, int L>
+// SPLIT: the launch may hold split jobs (row tiles on workgroups of their
+// own, handing boundaries over through device memory); without it the row
+// loop carries none of the hand-off code.  SISD: the launch reproduces the
+// reference's scalar engine (its end cell, clamped local gap openings and
+// traceback walk).  Both are per launch, so each is a kernel instance
+// rather than a test in every row.
+template <int SUB, typename ST, int K, int MAXT, int L, bool SPLIT, bool SISD>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<ST, K>::value)))
 PoaAlignKernel(const KArgs a) {
   constexpr int CPW = Cols<ST>::kPerWord;
@@ -915,7 +925,7 @@ PoaAlignKernel(const KArgs a) {
   const int wave = tid >> 6;
   const int nwaves = nthreads >> 6;
   const int type = prm.type;
-  const bool sisd = prm.sisd != 0;
+  constexpr bool sisd = SISD;
   const int32_t m = prm.m, mis = prm.mis, g = prm.g, e = prm.e, q = prm.q, cc = prm.c;
   const int32_t pad = prm.pad;
   const uint32_t G2 = Splat<ST>(g), E2 = Splat<ST>(e), Q2 = Splat<ST>(q), C2 = Splat<ST>(cc);
@@ -947,7 +957,7 @@ PoaAlignKernel(const KArgs a) {
     if (ix >= n_items) break;
     // the job and, for a split job, the row tile this workgroup aligns
     uint32_t jx = ix, tile = 0;
-    if (a.items) {
+    if (SPLIT && a.items) {  // (a launch without split jobs: item ix is (ix, 0))
       const uint2 it = a.items[ix];
       jx = Uniform(it.x);
       tile = Uniform(it.y);
@@ -992,7 +1002,7 @@ PoaAlignKernel(const KArgs a) {
     // A split tile t > 0 receives, per row, the H of tile t - 1's last
     // column and the carry entering its first column through hand-off words
     // (xin); a tile before the last sends them (xout).
-    const bool split = J.split > 1;
+    const bool split = SPLIT && J.split > 1;
     const uint32_t t_begin = split ? tile : 0u, t_end = split ? tile + 1 : ntiles;
     const bool single = t_end - t_begin == 1;  // one tile a row: the previous row stays in registers
     unsigned long long* xin = nullptr;
@@ -1924,7 +1934,7 @@ PoaAlignKernel(const KArgs a) {
         int32_t status = kDevOk;
         const unsigned long long tt0 = SPOA_PH_NOW();
         unsigned long long tph[kTbCount] = {};
-        Traceback<SUB, ST>(prm, gv, seq, n, mat, rstride, stride, max_i, max_j, a.pairs + 2ull * J.pair_off,
+        Traceback<SUB, ST, SISD>(prm, gv, seq, n, mat, rstride, stride, max_i, max_j, a.pairs + 2ull * J.pair_off,
                            J.cap, &np, &status, (Lds<uint32_t>*)(s_prof),
                            a.lds_bytes, lane, tph);
 #ifdef SPOA_ROW_PHASES
