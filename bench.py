@@ -365,9 +365,12 @@ def main():
     for _ in range(args.warmup):
         step()
     stats = []
+    step_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        ts = time.perf_counter()
         res = step()
+        step_ms.append(1000.0 * (time.perf_counter() - ts))
         stats.append(res.stats)
     el = time.perf_counter() - t0
     ms_per_step = 1000.0 * el / args.steps
@@ -377,6 +380,8 @@ def main():
     kernel_ms = sum(s["kernel_ms"] for s in stats)
     launches = sum(s["launches"] for s in stats)
     roof = dominant_kernel_roofline(ctx, stats, params, windows)
+    # the per-call engines below get the device memory the batched form held
+    ctx.close()
 
     # plain per-call drop-in on the first window (same bytes)
     per_call = None
@@ -433,6 +438,7 @@ def main():
         "roofline": dict(roof, traffic=measured_traffic(cfg["name"], n_windows),
                          all_launches={"launches": launches, "avg_launch_ms": kernel_ms / max(1, launches),
                                        "bytes_per_launch": dp_bytes / max(1, launches)}),
+        "step_ms": [round(x, 1) for x in step_ms],
         "breakdown_ms_per_step": {k: last[k] for k in ("wall_ms", "align_ms", "kernel_ms", "h2d_ms",
                                                        "d2h_ms", "pack_ms", "alloc_ms", "wait_ms",
                                                        "unpack_ms", "graph_ms", "finish_ms")},

@@ -523,6 +523,11 @@ class BatchContext:
             raise ValueError(_error())
 
     def __del__(self):
+        self.close()
+
+    def close(self):
+        """Releases the context's engines, device memory and host threads
+        now (the context is unusable afterwards)."""
         if getattr(self, "_h", None) and _lib is not None:
             _lib.spoa_batch_context_free(self._h)
             self._h = None

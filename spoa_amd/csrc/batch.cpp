@@ -474,6 +474,11 @@ void PerCallWindows(const WindowSet& ws, const BatchOptions& opt, std::uint32_t 
                     std::vector<WindowResult>& out, BatchStats& st) {
   auto engine = spoa::CreateEngine(opt.type, opt.m, opt.n, opt.g, opt.e, opt.q, opt.c, opt.sisd);
   auto* hip = dynamic_cast<spoa::HipAlignmentEngine*>(engine.get());
+  // the engines of the threads share the device's memory: each takes its
+  // share of the budget an engine gets alone
+  if (hip && step > 1) {
+    spoa_hip_set_memory_budget(hip->device(), spoa_hip_memory_budget(hip->device()) / step);
+  }
   for (std::uint32_t w = first; w < ws.n_windows; w += step) {
     spoa::Graph graph;
     for (std::uint64_t k = ws.win_read_off[w]; k < ws.win_read_off[w + 1]; ++k) {
