@@ -31,6 +31,7 @@ import concurrent.futures
 import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -42,16 +43,16 @@ import spoa_amd  # noqa: E402
 PEAK_HBM_GBS = 8000.0
 
 
-def measured_traffic(workload, n_windows):
+def measured_traffic(workload, n_windows, kernel):
     """HBM bytes per alignment launch measured by rocprofv3 for this
-    workload (profiles/traffic.json), or None."""
+    workload and kernel (profiles/traffic.json), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f).get(workload)
     except (OSError, ValueError):
         return None
-    if not rec or rec.get("windows_per_step") != n_windows:
+    if not rec or rec.get("windows_per_step") != n_windows or rec.get("kernel") != kernel:
         return None
     return rec.get("traffic_bytes_per_launch")
 
@@ -75,11 +76,14 @@ def dominant_kernel_roofline(ctx, stats, params, windows):
     sub = spoa_amd.AlignmentEngine.create(*params).subtype
     narrow = w == 32 and sub == 2 and params[0] == spoa_amd.SW
     bits = 16 if w == 16 or narrow else 32
-    k, waves = ctypes.c_int32(0), ctypes.c_int32(0)
-    spoa_amd.lib().spoa_hip_config(ctx.device, max(1, windows.max_read_len()), w, 1, ctypes.byref(k),
-                                   ctypes.byref(waves))
-    name = "PoaAlignKernel<%d, %s, %d, %d, %d>" % (sub, "short" if bits == 16 else "int", k.value,
-                                                    64 if waves.value == 1 else 512, 16 if w == 16 else 8)
+    k, waves, split = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+    spoa_amd.lib().spoa_hip_config_split(ctx.device, max(1, windows.max_read_len()), w, 1, ctypes.byref(k),
+                                         ctypes.byref(waves), ctypes.byref(split))
+    # template arguments: subtype, stored type, columns a lane, workgroup
+    # size, reference lanes, split-capable, scalar-engine rules
+    name = "PoaAlignKernel<%d, %s, %d, %d, %d, %s, false>" % (
+        sub, "short" if bits == 16 else "int", k.value, 64 if waves.value == 1 else 512, 16 if w == 16 else 8,
+        "true" if split.value else "false")
     return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS, "kernel": name, "score_bits": w, "launches": n,
             "avg_launch_ms": ms / max(1, n), "bytes_per_launch": byts / max(1, n)}
@@ -309,6 +313,40 @@ def serving_bench(args, threads):
     print(json.dumps(out))
 
 
+def percall_threads_child(args, threads):
+    """The per-call drop-in from `threads` host threads, one engine each
+    (the reference's threading model), over the first windows of the
+    workload; prints one JSON record with every window's consensus."""
+    cfg = spoa_amd.CONFIGS[args.config]
+    n_windows = args.windows or {1: 1024, 2: 2048, 3: 4096, 4: 256}[args.config]
+    windows = spoa_amd.config_windows(args.config, args.seed, n_windows)
+    some = windows.subset(list(range(args.percall_threads_child)))
+    t0 = time.perf_counter()
+    pm = spoa_amd.run_windows(some, *cfg["params"], mode="per_call", strand=cfg.get("strand", False), msa=True,
+                              threads=threads)
+    el = time.perf_counter() - t0
+    print(json.dumps({"threads": threads, "windows": some.n_windows, "windows_per_s": some.n_windows / el,
+                      "gcups": pm.stats["cells"] / (el * 1e9), "consensus": list(pm.consensus)}))
+
+
+def percall_threads(args, nt, nw, res):
+    """Runs percall_threads_child in a child process whose HIP runtime has
+    as many hardware queues as threads (GPU_MAX_HW_QUEUES; HIP's default of
+    4 would serialise the engines' streams beyond four), as a multi-threaded
+    caller of the drop-in deploys it."""
+    env = dict(os.environ, GPU_MAX_HW_QUEUES=str(max(4, nt)))
+    cmd = [sys.executable, os.path.abspath(__file__), "--config", str(args.config), "--seed", str(args.seed),
+           "--threads", str(nt), "--percall-threads-child", str(nw)]
+    if args.windows:
+        cmd += ["--windows", str(args.windows)]
+    out = subprocess.run(cmd, env=env, check=True, stdout=subprocess.PIPE, text=True).stdout
+    rec = json.loads(out.strip().split("\n")[-1])
+    cons = rec.pop("consensus")
+    rec["hw_queues"] = max(4, nt)
+    rec["same_consensus_as_batched"] = all(cons[w] == res.consensus[w] for w in range(len(cons)))
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -330,12 +368,13 @@ def main():
                     help="where window graphs live and are merged: host threads, or device-resident "
                          "(auto: host for the 1.5 kb local convex windows, whose 9k-node graphs sort faster "
                          "on 16 host threads beside the alignments; device for the others)")
+    ap.add_argument("--percall-threads-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(windows shard across GPUs without any exchange, see DESIGN.md)" % args.gpus,
+              "(windows would shard across GPUs without any exchange between them)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
@@ -343,6 +382,9 @@ def main():
         copy_probe(args.copy_probe << 20)
 
     threads = args.threads or min(16, os.cpu_count() or 1)
+    if args.percall_threads_child:
+        percall_threads_child(args, threads)
+        return
     if args.config == 5:
         serving_bench(args, threads)
         return
@@ -400,14 +442,7 @@ def main():
         nt = min(threads, n_windows)
         per_thread = {1: 8, 2: 2, 3: 4, 4: 1}[args.config]
         nw = min(n_windows, nt * per_thread)
-        some = windows.subset(list(range(nw)))
-        t2 = time.perf_counter()
-        pm = spoa_amd.run_windows(some, *params, mode="per_call", strand=strand, msa=True, threads=nt)
-        mel = time.perf_counter() - t2
-        per_call["threads"] = {"threads": nt, "windows": nw, "windows_per_s": nw / mel,
-                               "gcups": pm.stats["cells"] / (mel * 1e9),
-                               "same_consensus_as_batched": all(pm.consensus[w] == res.consensus[w]
-                                                                for w in range(nw))}
+        per_call["threads"] = percall_threads(args, nt, nw, res)
 
     cpu = None
     if not args.no_cpu:
@@ -435,7 +470,7 @@ def main():
                    "graphs": "device-resident" if merge == "device" else "host"},
         "gcups_dp": cells / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else 0.0,
         "gcups_end_to_end": cells / el / 1e9,
-        "roofline": dict(roof, traffic=measured_traffic(cfg["name"], n_windows),
+        "roofline": dict(roof, traffic=measured_traffic(cfg["name"], n_windows, roof["kernel"]),
                          all_launches={"launches": launches, "avg_launch_ms": kernel_ms / max(1, launches),
                                        "bytes_per_launch": dp_bytes / max(1, launches)}),
         "step_ms": [round(x, 1) for x in step_ms],
