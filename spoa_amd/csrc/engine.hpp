@@ -337,6 +337,17 @@ struct spoa_hip_engine {
   DevBuf d_xfer;
   uint32_t epoch = 0;
   uint32_t wait_ticks = 1000000000u;
+  // Buffers of the last device-resident window set the engine ran
+  // (spoa_hip_windows_destroy hands them back, the next
+  // spoa_hip_windows_create takes them over), so that repeated runs of
+  // window sets allocate device and pinned memory only to grow it.
+  struct WinBuffers {
+    DevBuf d[12];
+    PinBuf h[5];
+    std::vector<hipEvent_t> ev_s, ev_e;
+    hipEvent_t ev_m0 = nullptr, ev_m1 = nullptr, ev_done = nullptr;
+    bool held = false;
+  } win_bufs;
   // DP matrices: grow-only segments kept for the engine's lifetime, so a
   // long-lived engine allocates once (allocation time grows with size)
   std::vector<DevBuf> arena;
@@ -359,6 +370,22 @@ struct spoa_hip_engine {
 };
 
 namespace spoa_hip_host {
+
+// Frees the window-set buffers and events an engine keeps (WinBuffers).
+inline void ReleaseWinBuffers(spoa_hip_engine* E) {
+  auto& c = E->win_bufs;
+  for (auto& b : c.d) b.Release();
+  for (auto& b : c.h) b.Release();
+  for (auto ev : c.ev_s) (void)hipEventDestroy(ev);
+  for (auto ev : c.ev_e) (void)hipEventDestroy(ev);
+  c.ev_s.clear();
+  c.ev_e.clear();
+  for (hipEvent_t* ev : {&c.ev_m0, &c.ev_m1, &c.ev_done}) {
+    if (*ev) (void)hipEventDestroy(*ev);
+    *ev = nullptr;
+  }
+  c.held = false;
+}
 
 
 // Batched launches: 16 columns a lane for int16 rows (at most 1,024

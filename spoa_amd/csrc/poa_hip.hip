@@ -1147,6 +1147,7 @@ int spoa_hip_engine_destroy(spoa_hip_engine* E) {
   (void)hipSetDevice(E->device);
   if (E->pending.active) (void)RunPassFinish(E);
   if (E->stream) (void)hipStreamSynchronize(E->stream);
+  ReleaseWinBuffers(E);
   E->d_in.Release();
   E->d_out.Release();
   for (auto& sg : E->arena) sg.Release();

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdint>
 #include <cstring>
@@ -280,7 +281,7 @@ struct spoa_hip_windows {
   std::vector<WinInfo> info;
   DevBuf d_windows, d_blocks, d_seqs, d_quals, d_info, d_summary;
   DevBuf d_table, d_res, d_pairs, d_counter, d_out, d_offs;
-  PinBuf h_table, h_summary, h_out, h_offs;
+  PinBuf h_table, h_summary, h_out, h_offs, h_stage;
   std::vector<WinSummary> state;  // per window, as of the last finished round
   // the submitted round
   bool pending = false;
@@ -300,6 +301,41 @@ struct spoa_hip_windows {
 };
 
 namespace {
+
+// The window set's buffers and events, in the order of the engine's
+// WinBuffers cache.
+std::array<DevBuf*, 12> DevBufsOf(spoa_hip_windows* X) {
+  return {&X->d_windows, &X->d_blocks, &X->d_seqs, &X->d_quals, &X->d_info, &X->d_summary,
+          &X->d_table,   &X->d_res,    &X->d_pairs, &X->d_counter, &X->d_out, &X->d_offs};
+}
+std::array<PinBuf*, 5> PinBufsOf(spoa_hip_windows* X) {
+  return {&X->h_table, &X->h_summary, &X->h_out, &X->h_offs, &X->h_stage};
+}
+
+// Takes over the buffers the engine kept from its last window set.
+void AdoptBuffers(spoa_hip_windows* X) {
+  auto& c = X->E->win_bufs;
+  if (!c.held) return;
+  auto d = DevBufsOf(X);
+  for (size_t i = 0; i < d.size(); ++i) {
+    *d[i] = c.d[i];
+    c.d[i] = DevBuf{};
+  }
+  auto h = PinBufsOf(X);
+  for (size_t i = 0; i < h.size(); ++i) {
+    *h[i] = c.h[i];
+    c.h[i] = PinBuf{};
+  }
+  X->ev_s.swap(c.ev_s);
+  X->ev_e.swap(c.ev_e);
+  c.ev_s.clear();
+  c.ev_e.clear();
+  X->ev_m0 = c.ev_m0;
+  X->ev_m1 = c.ev_m1;
+  X->ev_done = c.ev_done;
+  c.ev_m0 = c.ev_m1 = c.ev_done = nullptr;
+  c.held = false;
+}
 
 int WinFail(spoa_hip_windows* X, int code, const std::string& msg) {
   X->last_error = msg;
@@ -373,6 +409,7 @@ int spoa_hip_windows_create(spoa_hip_engine* E, uint32_t n_windows, const uint32
   }
   auto* X = new spoa_hip_windows();
   X->E = E;
+  AdoptBuffers(X);
   X->n = n_windows;
   X->strand = strand != 0;
   X->quals = quals != nullptr;
@@ -447,7 +484,7 @@ int spoa_hip_windows_create(spoa_hip_engine* E, uint32_t n_windows, const uint32
   }
   // reads (and reverse complements, qualities) in one staging copy
   {
-    PinBuf stage;
+    PinBuf& stage = X->h_stage;
     if (!stage.Reserve(std::max<uint64_t>(sbytes, 256))) {
       spoa_hip_windows_destroy(X);
       return SPOA_HIP_DEVICE_ERROR;
@@ -472,7 +509,6 @@ int spoa_hip_windows_create(spoa_hip_engine* E, uint32_t n_windows, const uint32
       ok = HipOk(hipMemcpyAsync(X->d_quals.p, h, qbytes, hipMemcpyHostToDevice, E->stream), "copy in") &&
            HipOk(hipStreamSynchronize(E->stream), "copy in");
     }
-    stage.Release();
     if (ok) {
       ok = HipOk(hipMemcpyAsync(X->d_info.p, X->info.data(), n_windows * sizeof(WinInfo), hipMemcpyHostToDevice,
                                 E->stream),
@@ -490,8 +526,9 @@ int spoa_hip_windows_create(spoa_hip_engine* E, uint32_t n_windows, const uint32
       return SPOA_HIP_DEVICE_ERROR;
     }
   }
-  if (!HipOk(hipEventCreate(&X->ev_m0), "hipEventCreate") || !HipOk(hipEventCreate(&X->ev_m1), "hipEventCreate") ||
-      !HipOk(hipEventCreateWithFlags(&X->ev_done, hipEventDisableTiming), "hipEventCreate")) {
+  if ((!X->ev_m0 && !HipOk(hipEventCreate(&X->ev_m0), "hipEventCreate")) ||
+      (!X->ev_m1 && !HipOk(hipEventCreate(&X->ev_m1), "hipEventCreate")) ||
+      (!X->ev_done && !HipOk(hipEventCreateWithFlags(&X->ev_done, hipEventDisableTiming), "hipEventCreate"))) {
     spoa_hip_windows_destroy(X);
     return SPOA_HIP_DEVICE_ERROR;
   }
@@ -507,18 +544,24 @@ int spoa_hip_windows_create(spoa_hip_engine* E, uint32_t n_windows, const uint32
 
 int spoa_hip_windows_destroy(spoa_hip_windows* X) {
   if (!X) return SPOA_HIP_OK;
-  (void)hipSetDevice(X->E->device);
-  if (X->pending) (void)hipStreamSynchronize(X->E->stream);
-  for (DevBuf* b : {&X->d_windows, &X->d_blocks, &X->d_seqs, &X->d_quals, &X->d_info, &X->d_summary, &X->d_table,
-                    &X->d_res, &X->d_pairs, &X->d_counter, &X->d_out, &X->d_offs}) {
-    b->Release();
-  }
-  for (PinBuf* b : {&X->h_table, &X->h_summary, &X->h_out, &X->h_offs}) b->Release();
-  for (auto ev : X->ev_s) (void)hipEventDestroy(ev);
-  for (auto ev : X->ev_e) (void)hipEventDestroy(ev);
-  if (X->ev_m0) (void)hipEventDestroy(X->ev_m0);
-  if (X->ev_m1) (void)hipEventDestroy(X->ev_m1);
-  if (X->ev_done) (void)hipEventDestroy(X->ev_done);
+  spoa_hip_engine* E = X->E;
+  (void)hipSetDevice(E->device);
+  // nothing of the set may be in flight when the next set takes its buffers
+  (void)hipStreamSynchronize(E->stream);
+  // the engine keeps the buffers for its next window set (one set's worth:
+  // what it held before is freed)
+  ReleaseWinBuffers(E);
+  auto& c = E->win_bufs;
+  auto d = DevBufsOf(X);
+  for (size_t i = 0; i < d.size(); ++i) c.d[i] = *d[i];
+  auto h = PinBufsOf(X);
+  for (size_t i = 0; i < h.size(); ++i) c.h[i] = *h[i];
+  c.ev_s.swap(X->ev_s);
+  c.ev_e.swap(X->ev_e);
+  c.ev_m0 = X->ev_m0;
+  c.ev_m1 = X->ev_m1;
+  c.ev_done = X->ev_done;
+  c.held = true;
   delete X;
   return SPOA_HIP_OK;
 }
