@@ -368,6 +368,9 @@ def main():
                     help="where window graphs live and are merged: host threads, or device-resident "
                          "(auto: host for the 1.5 kb local convex windows, whose 9k-node graphs sort faster "
                          "on 16 host threads beside the alignments; device for the others)")
+    ap.add_argument("--graph", action="store_true",
+                    help="launch every pass of the host-merged batched form as a HIP graph (updated in place "
+                         "from pass to pass)")
     ap.add_argument("--percall-threads-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates HBM byte counters)")
@@ -395,7 +398,8 @@ def main():
     windows = spoa_amd.config_windows(args.config, args.seed, n_windows)
 
     merge = args.merge if args.merge != "auto" else ("host" if args.config == 2 else "device")
-    ctx = spoa_amd.BatchContext(*params, strand=strand, msa=True, threads=threads, resident=merge == "device")
+    ctx = spoa_amd.BatchContext(*params, strand=strand, msa=True, threads=threads, resident=merge == "device",
+                                graph=args.graph)
     tuned = None
     if not args.no_tune:
         # launch configuration per score width from the autotuner (untimed)
@@ -467,7 +471,8 @@ def main():
         "data": "synthetic (seeded uniform ACGT templates, mixed substitution/insertion/deletion errors)",
         "config": {"workload": cfg["name"], "windows_per_step": n_windows, "seed": args.seed,
                    "params": list(params), "strand": strand, "host_threads": threads,
-                   "graphs": "device-resident" if merge == "device" else "host"},
+                   "graphs": "device-resident" if merge == "device" else "host",
+                   "hip_graph": bool(args.graph)},
         "gcups_dp": cells / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else 0.0,
         "gcups_end_to_end": cells / el / 1e9,
         "roofline": dict(roof, traffic=measured_traffic(cfg["name"], n_windows, roof["kernel"]),

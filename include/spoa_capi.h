@@ -119,6 +119,7 @@ typedef struct spoa_batch_stats {
   double kernel_ms16, kernel_ms32; /* kernel_ms of the int16 / int32 launches */
   uint64_t dp_bytes16, dp_bytes32; /* dp_bytes of the int16 / int32 jobs */
   uint32_t launches16, launches32; /* launches of int16 / int32 jobs */
+  uint32_t graph_builds, graph_updates; /* graph mode: passes instantiated / updated in place and replayed */
 } spoa_batch_stats;
 
 /* Runs every window (reads [win_read_off[w], win_read_off[w+1]); read k is

@@ -172,6 +172,10 @@ typedef struct spoa_hip_stats {
   uint32_t launches16, launches32;  /* launches of int16 / int32 jobs */
   double merge_ms;   /* device-resident windows: device time of the graph merges */
   uint32_t merges;   /* device-resident windows: windows merged */
+  /* graph mode: passes whose capture was instantiated as a new executable
+   * graph, and passes that updated the previous pass's executable graph in
+   * place (same shape) and replayed it */
+  uint32_t graph_builds, graph_updates;
 } spoa_hip_stats;
 int spoa_hip_last_stats(spoa_hip_engine* engine, spoa_hip_stats* out);
 

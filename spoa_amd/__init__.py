@@ -50,7 +50,8 @@ class HipStats(ctypes.Structure):
                 ("kernel_ms16", ctypes.c_double), ("kernel_ms32", ctypes.c_double),
                 ("dp_bytes16", ctypes.c_uint64), ("dp_bytes32", ctypes.c_uint64),
                 ("launches16", ctypes.c_uint32), ("launches32", ctypes.c_uint32),
-                ("merge_ms", ctypes.c_double), ("merges", ctypes.c_uint32)]
+                ("merge_ms", ctypes.c_double), ("merges", ctypes.c_uint32),
+                ("graph_builds", ctypes.c_uint32), ("graph_updates", ctypes.c_uint32)]
 
 
 class BatchOpts(ctypes.Structure):
@@ -70,7 +71,8 @@ class BatchStats(ctypes.Structure):
                 ("launches", ctypes.c_uint32), ("rounds", ctypes.c_uint32),
                 ("kernel_ms16", ctypes.c_double), ("kernel_ms32", ctypes.c_double),
                 ("dp_bytes16", ctypes.c_uint64), ("dp_bytes32", ctypes.c_uint64),
-                ("launches16", ctypes.c_uint32), ("launches32", ctypes.c_uint32)]
+                ("launches16", ctypes.c_uint32), ("launches32", ctypes.c_uint32),
+                ("graph_builds", ctypes.c_uint32), ("graph_updates", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -242,6 +242,8 @@ std::vector<WindowResult> RunResident(const WindowSet& ws, BatchContext& ctx, Ba
     st.kernel_ms += hs.kernel_ms;
     st.kernel_ms16 += hs.kernel_ms16;
     st.kernel_ms32 += hs.kernel_ms32;
+    st.graph_builds += hs.graph_builds;
+    st.graph_updates += hs.graph_updates;
     st.dp_bytes16 += hs.dp_bytes16;
     st.dp_bytes32 += hs.dp_bytes32;
     st.launches16 += hs.launches16;
@@ -369,6 +371,8 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
     st.kernel_ms += hs.kernel_ms;
     st.kernel_ms16 += hs.kernel_ms16;
     st.kernel_ms32 += hs.kernel_ms32;
+    st.graph_builds += hs.graph_builds;
+    st.graph_updates += hs.graph_updates;
     st.dp_bytes16 += hs.dp_bytes16;
     st.dp_bytes32 += hs.dp_bytes32;
     st.launches16 += hs.launches16;

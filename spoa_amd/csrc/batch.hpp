@@ -59,6 +59,7 @@ struct BatchStats {
   double align_ms = 0;       // inside the alignment calls (pack + copies + kernels + unpack)
   double kernel_ms = 0;      // alignment kernels (HIP events)
   double kernel_ms16 = 0, kernel_ms32 = 0;  // of the int16 / int32 launches
+  std::uint32_t graph_builds = 0, graph_updates = 0;  // graph mode: passes instantiated / updated in place
   std::uint64_t dp_bytes16 = 0, dp_bytes32 = 0;  // of the int16 / int32 jobs
   std::uint32_t launches16 = 0, launches32 = 0;
   double h2d_ms = 0, d2h_ms = 0;

@@ -469,6 +469,8 @@ int spoa_batch_get_stats(void* h, spoa_batch_stats* out) {
   out->launches = s.launches;
   out->rounds = s.rounds;
   out->kernel_ms16 = s.kernel_ms16;
+  out->graph_builds = s.graph_builds;
+  out->graph_updates = s.graph_updates;
   out->kernel_ms32 = s.kernel_ms32;
   out->dp_bytes16 = s.dp_bytes16;
   out->dp_bytes32 = s.dp_bytes32;

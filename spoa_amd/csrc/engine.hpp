@@ -316,8 +316,7 @@ struct spoa_hip_engine {
   // graph mode: every pass's launch sequence is captured into a HIP graph
   // and launched as one unit on the compute stream
   bool graph_mode = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  hipGraphExec_t graph_exec = nullptr;  // kept from pass to pass, updated in place
   bool graph_pass = false;  // the pending pass ran as a graph
   std::vector<int> pass_width;  // score width of every launch of the pass
   spoa_hip_params prm{};

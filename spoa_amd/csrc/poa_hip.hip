@@ -771,9 +771,27 @@ int RunPassLaunch(spoa_hip_engine* E, spoa_hip_job* jobs, std::vector<Plan>&& pl
       if (gr) (void)hipGraphDestroy(gr);
       return SPOA_HIP_DEVICE_ERROR;
     }
-    E->graph = gr;
-    if (!HipOk(hipGraphInstantiate(&E->graph_exec, gr, nullptr, nullptr, 0), "graph instantiate") ||
-        !HipOk(hipEventRecord(E->ev_ks[0], cs), "event") || !HipOk(hipGraphLaunch(E->graph_exec, cs), "graph launch") ||
+    // The previous pass's executable graph takes this pass's nodes in place
+    // (same kernels and copies; new grid sizes, arguments and byte counts)
+    // when the two captures have the same shape; otherwise the capture is
+    // instantiated anew.  The previous launch of it has finished (its pass
+    // was waited for before this one was built).
+    bool updated = false;
+    if (E->graph_exec) {
+      hipGraphNode_t err_node = nullptr;
+      hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+      if (hipGraphExecUpdate(E->graph_exec, gr, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+        updated = true;
+      } else {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(E->graph_exec);
+        E->graph_exec = nullptr;
+      }
+    }
+    const bool built = updated || HipOk(hipGraphInstantiate(&E->graph_exec, gr, nullptr, nullptr, 0), "graph instantiate");
+    (void)hipGraphDestroy(gr);
+    if (built) ++(updated ? E->stats.graph_updates : E->stats.graph_builds);
+    if (!built || !HipOk(hipEventRecord(E->ev_ks[0], cs), "event") || !HipOk(hipGraphLaunch(E->graph_exec, cs), "graph launch") ||
         !HipOk(hipEventRecord(E->ev_ke[0], cs), "event") || !HipOk(hipEventRecord(E->ev_kdone, cs), "event")) {
       return SPOA_HIP_DEVICE_ERROR;
     }
@@ -860,12 +878,8 @@ int RunPassFinish(spoa_hip_engine* E) {
   hipStream_t st = E->copy;
   const auto t_wait0 = std::chrono::steady_clock::now();
   if (E->graph_pass) {
-    const bool ok = HipOk(hipEventSynchronize(E->ev_kdone), "graph");
-    if (E->graph_exec) (void)hipGraphExecDestroy(E->graph_exec);
-    if (E->graph) (void)hipGraphDestroy(E->graph);
-    E->graph_exec = nullptr;
-    E->graph = nullptr;
-    if (!ok) return SPOA_HIP_DEVICE_ERROR;
+    // the executable graph stays for the next pass to update
+    if (!HipOk(hipEventSynchronize(E->ev_kdone), "graph")) return SPOA_HIP_DEVICE_ERROR;
   } else if (!HipOk(hipStreamSynchronize(st), "kernel")) {
     return SPOA_HIP_DEVICE_ERROR;
   }
@@ -1160,7 +1174,6 @@ int spoa_hip_engine_destroy(spoa_hip_engine* E) {
   if (E->ev2) (void)hipEventDestroy(E->ev2);
   if (E->ev3) (void)hipEventDestroy(E->ev3);
   if (E->graph_exec) (void)hipGraphExecDestroy(E->graph_exec);
-  if (E->graph) (void)hipGraphDestroy(E->graph);
   if (E->ev_in) (void)hipEventDestroy(E->ev_in);
   if (E->ev_kdone) (void)hipEventDestroy(E->ev_kdone);
   if (E->ev_side_in) (void)hipEventDestroy(E->ev_side_in);

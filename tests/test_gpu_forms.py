@@ -237,8 +237,9 @@ def test_autotuner_picks_configs_and_keeps_results(tmp_path):
 
 def test_graph_mode_replays_match_stream_mode():
     """Every pass launched as a captured HIP graph: the same results as the
-    stream launches, run after run (each run replays new graphs), with
-    eager stream-mode runs in between."""
+    stream launches, run after run, with eager stream-mode runs in between.
+    Passes of the same shape as the engine's previous one update its
+    executable graph in place and replay it."""
     c = spoa_amd.CONFIGS[2]
     ws = spoa_amd.config_windows(2, seed=8, n_windows=12)
     eager = spoa_amd.BatchContext(*c["params"], msa=True)
@@ -248,6 +249,10 @@ def test_graph_mode_replays_match_stream_mode():
         res = graph.run(ws)
         assert res.consensus == ref.consensus
         assert res.msa == ref.msa
+        assert res.stats["graph_updates"] > 0
+        # a pass per half per round, but for the first round (every read
+        # then meets an empty graph: nothing to launch)
+        assert res.stats["graph_builds"] + res.stats["graph_updates"] == (res.stats["rounds"] - 1) * 2
         again = eager.run(ws)
         assert again.consensus == ref.consensus
 

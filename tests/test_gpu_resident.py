@@ -195,3 +195,24 @@ def test_resident_large_window_sorts_in_device_memory():
     res = spoa_amd.run_windows(windows, *params, mode="resident", msa=True)
     assert res.consensus == host.consensus
     assert res.msa == host.msa
+
+
+def test_context_reuses_buffers_across_window_sets():
+    """One resident context runs a larger set, a smaller and different one
+    (strand mode, other read lengths), then the larger one again: each run
+    takes over the buffers the last one left and equals the oracle run
+    alone on every window (and the first run, for the repeat)."""
+    params = (ol.NW, 5, -4, -8, -6, -10, -4)
+    big = _windows(21, 12, tlen=(300, 700), nreads=(5, 14), p=0.12, revcomp=True)
+    small = _windows(22, 5, tlen=(60, 200), nreads=(2, 6), p=0.15, revcomp=True)
+    ctx = spoa_amd.BatchContext(*params, strand=True, msa=True, resident=True)
+    first = ctx.run(big)
+    second = ctx.run(small)
+    third = ctx.run(big)
+    for windows, res in ((big, first), (small, second)):
+        for w, reads in enumerate(windows):
+            cons, msa, _ = ol.run_window(ol.oracle(), params, reads, strand=True)
+            assert res.consensus[w] == cons and res.msa[w] == msa, w
+    assert list(third.consensus) == list(first.consensus)
+    assert [third.msa[w] for w in range(len(big))] == [first.msa[w] for w in range(len(big))]
+    ctx.close()
