@@ -4,8 +4,9 @@
 // whether the launch holds split jobs or reproduces the scalar engine.
 //
 // The instances of one (subtype, stored width, lane count) family are built
-// in a translation unit of their own (poa_k_*.hip), so the families compile
-// in parallel; each unit defines its family's PickFamily specialisation.
+// in a translation unit of their own (poa_family.hip compiled once per
+// family), so the families compile in parallel; each unit defines its
+// family's PickFamily specialisation.
 #pragma once
 
 #include "engine.hpp"
