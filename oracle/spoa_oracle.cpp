@@ -1060,7 +1060,7 @@ int oracle_align(void* eng, void* graph, const char* seq, std::uint32_t len,
                      : oracle::Align(en, *static_cast<Graph*>(graph), seq, len);
     std::uint32_t np = static_cast<std::uint32_t>(r.pairs.size() / 2);
     if (np > cap) return -2;
-    std::memcpy(pairs_out, r.pairs.data(), r.pairs.size() * sizeof(std::int32_t));
+    if (np > 0) std::memcpy(pairs_out, r.pairs.data(), r.pairs.size() * sizeof(std::int32_t));
     if (has_score) *has_score = r.has_score ? 1 : 0;
     if (r.has_score && score) *score = r.score;
     if (width_bits) *width_bits = r.width_bits;

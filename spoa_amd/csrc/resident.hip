@@ -47,6 +47,31 @@ struct WaveTeam {
     return x - v;
   }
   __device__ bool any(bool b) const { return __ballot(b) != 0; }
+  // lane 0's value, held in a scalar register: branches on it are scalar
+  __device__ uint32_t uniform(uint32_t v) const {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+  }
+  // the lanes where b holds, as a bit mask
+  __device__ uint64_t ballot(bool b) const { return __ballot(b); }
+  // set bits of m below this lane
+  __device__ uint32_t below(uint64_t m) const {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+  }
+  // lane l's value (l uniform)
+  __device__ uint32_t readlane(uint32_t v, uint32_t l) const {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(l)));
+  }
+  // the minimum over lanes 0-15, uniform: a DPP prefix minimum within the
+  // first row of 16 lanes, read from its last lane
+  __device__ uint32_t min16(uint32_t v) const {
+    int32_t x = static_cast<int32_t>(v);
+    x = static_cast<int32_t>(min(static_cast<uint32_t>(x), static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x111, 0xf, 0xf, false))));
+    x = static_cast<int32_t>(min(static_cast<uint32_t>(x), static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x112, 0xf, 0xf, false))));
+    x = static_cast<int32_t>(min(static_cast<uint32_t>(x), static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x114, 0xf, 0xf, false))));
+    x = static_cast<int32_t>(min(static_cast<uint32_t>(x), static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x118, 0xf, 0xf, false))));
+    return readlane(static_cast<uint32_t>(x), 15);
+  }
   __device__ uint32_t min_all(uint32_t v) const {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v = min(v, static_cast<uint32_t>(__shfl_xor(v, d, 64)));
@@ -119,7 +144,7 @@ __device__ void Summarise(const wg::WinView& g, WinSummary* out, uint32_t A) {
 // (src/main.cpp:285-302, forward on ties), AddAlignment, TopologicalSort,
 // the next graph block.
 __global__ void __launch_bounds__(64) MergeKernel(MergeArgs a) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   const WaveTeam t{threadIdx.x};
   const MergeItem it = a.items[blockIdx.x];
   const WinInfo wi = a.info[it.window];

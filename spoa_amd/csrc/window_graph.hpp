@@ -43,6 +43,7 @@ namespace wg {
 constexpr uint32_t kNil = 0xffffffffu;
 constexpr uint32_t kMaxAligned = 7;  // aligned groups of up to 8 codes
 constexpr uint16_t kNoCode = 0xffffu;
+constexpr uint32_t kSortSink = 64;  // topological sort: a slot per lane for writes nothing reads
 
 enum WinStatus : int32_t {
   kWinOk = 0,
@@ -87,13 +88,14 @@ struct WinLayout {
 };
 
 // Scratch words the sort needs in global mode (32-bit indices): in-edge
-// offsets and tails by node id, aligned-list offsets and entries, marks (a
-// byte each), the stack (every push follows an in-edge or an aligned entry
-// of a node visited once, or starts a root) and, for global alignments, the
-// source distance of every node.
+// offsets and tails by node id, aligned-list offsets and entries, the stack
+// (every push follows an in-edge or an aligned entry of a node visited once,
+// or starts a root), for global alignments the source distance of every
+// node, the rank order, a sink, then the marks (a byte each) at an 8-byte
+// boundary and their sink.
 __host__ __device__ inline uint64_t SortScratchWords(uint64_t V, uint64_t E) {
   const uint64_t A = V * kMaxAligned;
-  return (V + 1) + E + (V + 1) + A + (V + 3) / 4 + (V + E + A + 1) + V + V;
+  return (V + 1) + E + (V + 1) + A + (V + E + A + 1) + V + V + kSortSink + 1 + (V + 7) / 8 * 2 + kSortSink / 4;
 }
 
 __host__ __device__ inline WinLayout MakeLayout(const WinCaps& c) {
@@ -515,22 +517,33 @@ __host__ __device__ int32_t AddAlignment(const Team& t, const WinView& g, const 
 
 // Arrays of the topological sort, in LDS (16-bit indices) or in the
 // window's global scratch (32-bit).
+
 template <typename Idx>
 struct SortSpace {
   Idx* in_off;   // [V + 1] in-edge offsets by node id
   Idx* in_tail;  // [E] in-edge tails, each node's in creation order
   Idx* al_off;   // [V + 1]
   Idx* al_list;  // [A]
-  uint8_t* marks;  // [V]: 1 visited, 2 finished, 4 ranked with its group's first member
-  Idx* stack;      // [K]
+  Idx* stack;    // [K]
   uint32_t K;
-  Idx* dist;  // [V] source distance by node id (global alignments only)
-  Idx* r2n;   // [V] rank_to_node (global alignments only)
+  Idx* dist;       // [V] source distance by node id (global alignments only)
+  Idx* r2n;        // [V] rank order
+  Idx* sink;       // [kSortSink]
+  uint8_t* marks;  // [V rounded up to 8], 8-byte aligned: 1 visited, 2 finished,
+                   // 4 ranked with its group's first member
+  uint8_t* sink_mk;  // [kSortSink]
 };
 
-// Words (of Idx) a SortSpace needs.
+// Words (of Idx) of a SortSpace's index arrays.
 __host__ __device__ inline uint64_t SortSpaceElems(uint64_t V, uint64_t E, uint64_t A, uint64_t K, bool nw) {
-  return (V + 1) + E + (V + 1) + A + K + (nw ? 2 * V : 0);
+  return (V + 1) + E + (V + 1) + A + K + (nw ? V : 0) + V + kSortSink;
+}
+
+// Bytes of a SortSpace from an 8-byte aligned base: the index arrays, then
+// the marks at the next 8-byte boundary, then the marks' sink.
+__host__ __device__ inline uint64_t SortSpaceBytes(uint64_t V, uint64_t E, uint64_t A, uint64_t K, bool nw,
+                                                   uint64_t idx_bytes) {
+  return (SortSpaceElems(V, E, A, K, nw) * idx_bytes + 7) / 8 * 8 + (V + 7) / 8 * 8 + kSortSink;
 }
 
 template <typename Idx>
@@ -550,16 +563,21 @@ __host__ __device__ inline SortSpace<Idx> CarveSortSpace(void* base, uint32_t V,
   p += K;
   s.K = K;
   s.dist = nw ? p : nullptr;
-  s.r2n = nw ? p + V : nullptr;
-  p += nw ? 2 * V : 0;
-  s.marks = reinterpret_cast<uint8_t*>(p);
+  p += nw ? V : 0;
+  s.r2n = p;
+  p += V;
+  s.sink = p;
+  const uint64_t idx_bytes = (SortSpaceElems(V, E, A, K, nw) * sizeof(Idx) + 7) / 8 * 8;
+  s.marks = static_cast<uint8_t*>(base) + idx_bytes;
+  s.sink_mk = s.marks + (V + 7) / 8 * 8;
   return s;
 }
 
-// Bytes of a SortSpace (marks after the Idx arrays).
-__host__ __device__ inline uint64_t SortSpaceBytes(uint64_t V, uint64_t E, uint64_t A, uint64_t K, bool nw,
-                                                   uint64_t idx_bytes) {
-  return SortSpaceElems(V, E, A, K, nw) * idx_bytes + V;
+// Bytes 0..7 of w that are zero, as the high bit of each such byte (exact:
+// no carry crosses a byte).
+__host__ __device__ inline uint64_t ZeroBytes(uint64_t w) {
+  constexpr uint64_t k7f = 0x7f7f7f7f7f7f7f7full;
+  return ~(((w & k7f) + k7f) | w | k7f);
 }
 
 // Graph::TopologicalSort (src/graph.cpp:249-303): depth first over node ids;
@@ -567,9 +585,22 @@ __host__ __device__ inline uint64_t SortSpaceBytes(uint64_t V, uint64_t E, uint6
 // another node's group, its aligned nodes) are; a group is ranked when its
 // first finished non-ignored member is, the member first, then its aligned
 // list.  A node back on top of the stack after its first visit has every
-// node that visit pushed finished above it, so it is finished then.  The
-// per-node lists are copied into `sp` first (team-parallel); the walk is
-// serial (lane 0).  Returns false when the stack of `sp` overflows.
+// node that visit pushed finished above it, so it is finished then.  For a
+// global alignment each node's source distance (0 without in-edges, else one
+// more than its nearest tail's) is set when it finishes, after its tails.
+//
+// The per-node lists are copied into `sp` first (team-parallel).  The walk
+// itself is one serial order, so on a wave its cost is the instructions a
+// visit issues (a lone wave issues one every four cycles) plus its dependent
+// reads.  Every lane runs it with the same node (its values made uniform, so
+// its branches are scalar), and a visit's list work is spread over the
+// lanes: lane k takes entry k of the node's in-edge tails followed by its
+// aligned nodes (the order the reference pushes them in), reads it and its
+// mark, and the unfinished ones go onto the stack in that order through one
+// ballot and a prefix count.  Lanes with nothing to write write to a sink
+// slot of their own instead of branching.  The rank order is kept in `sp`
+// and written out team-parallel at the end.  Returns false when the stack
+// of `sp` overflows.
 template <class Team, typename Idx>
 __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const SortSpace<Idx>& sp) {
   const uint32_t V = g.hdr->V, E = g.hdr->E;
@@ -607,71 +638,124 @@ __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const 
     for (uint32_t k = 0; k < cnt; ++k) sp.al_list[o + k] = static_cast<Idx>(g.al[v * kMaxAligned + k]);
   }
   t.sync();
+  const bool nw = sp.dist != nullptr;
+  uint8_t* const mk = sp.marks;
+  auto U = [&](uint32_t x) { return t.uniform(x); };
+  const uint32_t Vu = U(V), K = U(sp.K), lane = t.lane;
+  uint32_t r = 0, top = 0;
   bool ok = true;
-  if (t.lane == 0) {
-    uint8_t* const mk = sp.marks;
-    uint32_t r = 0, top = 0;
-    auto emit = [&](uint32_t id) {
-      g.rank_of[id] = r;
-      g.rank_to_node[r] = id;
-      if (sp.r2n) sp.r2n[r] = static_cast<Idx>(id);
-      ++r;
-    };
-    auto emit_group = [&](uint32_t id) {
-      emit(id);
-      const uint32_t a1 = sp.al_off[id + 1];
-      for (uint32_t a = sp.al_off[id]; a < a1; ++a) emit(sp.al_list[a]);
-    };
-    for (uint32_t root = 0; root < V && ok; ++root) {
-      if (mk[root] != 0) continue;
-      sp.stack[top++] = static_cast<Idx>(root);
+  // entry q of a node's combined list: its in-edge tails, then its aligned nodes
+  auto entry = [&](uint32_t q, uint32_t i0, uint32_t nt, uint32_t a0) {
+    const Idx* const list = q < nt ? sp.in_tail : sp.al_list;
+    return static_cast<uint32_t>(list[q < nt ? i0 + q : a0 + (q - nt)]);
+  };
+  // the smallest distance over lanes [0, cnt) (cnt > 0)
+  auto min_lanes = [&](uint32_t v, uint32_t cnt) { return cnt <= 16 ? t.min16(v) : U(t.min_all(v)); };
+  for (uint32_t base = 0; base < Vu && ok; base += 8) {
+    uint32_t j = 0;
+    while (ok && j < 8) {
+      // the next root: the first unmarked node at or after base + j
+      const uint64_t w = *reinterpret_cast<const uint64_t*>(mk + base);
+      const uint64_t wu = static_cast<uint64_t>(U(static_cast<uint32_t>(w >> 32))) << 32 |
+                          U(static_cast<uint32_t>(w));
+      uint64_t z = ZeroBytes(wu) & (~0ull << (8 * j));
+      if (Vu - base < 8) z &= (1ull << (8 * (Vu - base))) - 1;
+      if (z == 0) break;
+      j = static_cast<uint32_t>(__builtin_ctzll(z)) >> 3;
+      uint32_t cur = base + j;
+      ++j;
+      if (K == 0) {
+        ok = false;
+        break;
+      }
+      sp.stack[0] = static_cast<Idx>(cur);
+      top = 1;
       while (top > 0) {
-        const uint32_t curr = sp.stack[top - 1];
-        const uint8_t m = mk[curr];
-        if ((m & 3) == 1) {
-          mk[curr] = static_cast<uint8_t>(m | 2);
-          if ((m & 4) == 0) emit_group(curr);
-        } else if ((m & 2) == 0) {
-          bool valid = true;
-          const uint32_t i1 = sp.in_off[curr + 1];
-          for (uint32_t i = sp.in_off[curr]; i < i1; ++i) {
-            const uint32_t tl = sp.in_tail[i];
-            if ((mk[tl] & 2) == 0) {
-              if (top == sp.K) {
+        const uint32_t m = U(mk[cur]);
+        const uint32_t i0 = U(sp.in_off[cur]), i1 = U(sp.in_off[cur + 1]);
+        const uint32_t a0 = U(sp.al_off[cur]), a1 = U(sp.al_off[cur + 1]);
+        const uint32_t below = U(sp.stack[top >= 2 ? top - 2 : 0]);
+        if (m & 2) {  // finished since it was pushed
+          --top;
+          cur = below;
+          continue;
+        }
+        const bool ign = (m & 4) != 0;
+        const uint32_t nt = i1 - i0, n = nt + (ign ? 0u : a1 - a0);
+        uint32_t dmin = kNil;
+        if ((m & 1) == 0) {
+          // first visit: the unfinished tails, then (unless ignored) the
+          // unfinished aligned nodes, marked ignored, go onto the stack; the
+          // aligned nodes are also written at the ranks they take if the node
+          // finishes now (ranks from r on are not final yet)
+          bool pushed = false;
+          uint32_t last = 0;
+          for (uint32_t c = 0; c < n; c += t.size) {
+            const uint32_t q = c + lane;
+            const bool valid = q < n;
+            const uint32_t qq = valid ? q : c;
+            const uint32_t val = entry(qq, i0, nt, a0);
+            const uint32_t mv = mk[val];
+            const bool un = valid && (mv & 2) == 0;
+            const uint64_t pm = t.ballot(un);
+            if (pm != 0) {
+              const uint32_t cnt = static_cast<uint32_t>(__builtin_popcountll(pm));
+              if (top + cnt > K) {
                 ok = false;
                 break;
               }
-              sp.stack[top++] = static_cast<Idx>(tl);
-              valid = false;
+              *(un ? sp.stack + top + t.below(pm) : sp.sink + lane) = static_cast<Idx>(val);
+              *(un && qq >= nt ? mk + val : sp.sink_mk + lane) = static_cast<uint8_t>(mv | 4);
+              top += cnt;
+              last = t.readlane(val, 63u - static_cast<uint32_t>(__builtin_clzll(pm)));
+              pushed = true;
+            }
+            if (!ign) *(valid && qq >= nt ? sp.r2n + r + 1 + (qq - nt) : sp.sink + lane) = static_cast<Idx>(val);
+            if (nw && c < nt) {
+              const uint32_t dv = sp.dist[val];
+              const uint32_t dm = min_lanes(valid && qq < nt ? dv : kNil, nt - c);
+              dmin = dm < dmin ? dm : dmin;
             }
           }
           if (!ok) break;
-          if ((m & 4) == 0) {
-            const uint32_t a1 = sp.al_off[curr + 1];
-            for (uint32_t a = sp.al_off[curr]; a < a1; ++a) {
-              const uint32_t al = sp.al_list[a];
-              if ((mk[al] & 2) == 0) {
-                if (top == sp.K) {
-                  ok = false;
-                  break;
-                }
-                sp.stack[top++] = static_cast<Idx>(al);
-                mk[al] = static_cast<uint8_t>(mk[al] | 4);
-                valid = false;
-              }
-            }
-            if (!ok) break;
-          }
-          if (!valid) {
-            mk[curr] = static_cast<uint8_t>(m | 1);
+          if (pushed) {
+            mk[cur] = static_cast<uint8_t>(m | 1);
+            cur = last;
             continue;
           }
-          mk[curr] = static_cast<uint8_t>(m | 2);
-          if ((m & 4) == 0) emit_group(curr);
+        } else {
+          // back on top, every node it pushed finished: its tails' distances
+          // and the aligned nodes it ranks
+          for (uint32_t c = nw ? 0u : nt; c < n; c += t.size) {
+            const uint32_t q = c + lane;
+            const bool valid = q < n;
+            const uint32_t qq = valid ? q : c;
+            const uint32_t val = entry(qq, i0, nt, a0);
+            if (!ign) *(valid && qq >= nt ? sp.r2n + r + 1 + (qq - nt) : sp.sink + lane) = static_cast<Idx>(val);
+            if (nw && c < nt) {
+              const uint32_t dv = sp.dist[val];
+              const uint32_t dm = min_lanes(valid && qq < nt ? dv : kNil, nt - c);
+              dmin = dm < dmin ? dm : dmin;
+            }
+          }
+        }
+        mk[cur] = static_cast<uint8_t>(m | 2);
+        if (nw) sp.dist[cur] = static_cast<Idx>(nt > 0 ? dmin + 1 : 0u);
+        if (!ign) {
+          sp.r2n[r] = static_cast<Idx>(cur);
+          r += 1 + (a1 - a0);
         }
         --top;
+        cur = below;
       }
     }
+  }
+  t.sync();
+  // the rank order out to the window (rank_to_node and its inverse)
+  for (uint32_t q = t.lane; q < V; q += t.size) {
+    const uint32_t v = sp.r2n[q];
+    g.rank_to_node[q] = v;
+    g.rank_of[v] = q;
   }
   t.sync();
   return t.any(!ok) ? false : true;
@@ -683,8 +767,9 @@ __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const 
 // list, first-column H), node ids, predecessor offsets, first columns
 // (reference simd...:568-577, 590-599, 607-694) and predecessor rows (rank +
 // 1, in-edge order).  First columns of a global alignment follow the source
-// distance d of each node (the fewest edges from a source): the vertical
-// gap column is g + d * e (q + d * c), the linear H column (d + 1) * g.
+// distance d of each node (the fewest edges from a source, which
+// TopologicalSort leaves in sp.dist): the vertical gap column is g + d * e
+// (q + d * c), the linear H column (d + 1) * g.
 // Also sets the header's R, nslots and slot_chars.
 template <class Team, typename Idx>
 __host__ __device__ void BuildBlock(const Team& t, const WinView& g, const SortSpace<Idx>& sp, uint32_t* block,
@@ -701,24 +786,6 @@ __host__ __device__ void BuildBlock(const Team& t, const WinView& g, const SortS
   uint32_t* preds = block + 9ull * V + 4;
   const bool nw = type == 1;
   const uint32_t nslots = h.ncodes <= 4 ? h.ncodes : 0u;
-  // source distances in rank order (serial; global alignments only)
-  if (nw && t.lane == 0) {
-    for (uint32_t r = 0; r < V; ++r) {
-      const uint32_t v = sp.r2n[r];
-      const uint32_t i0 = sp.in_off[v], i1 = sp.in_off[v + 1];
-      uint32_t d = 0;
-      if (i1 > i0) {
-        d = 0xffffffffu;
-        for (uint32_t i = i0; i < i1; ++i) {
-          const uint32_t x = sp.dist[sp.in_tail[i]];
-          d = x < d ? x : d;
-        }
-        d += 1;
-      }
-      sp.dist[v] = static_cast<Idx>(d);
-    }
-  }
-  t.sync();
   uint32_t run = 0, rsum = 0;
   for (uint32_t base = 0; base < V; base += t.size) {
     const uint32_t r = base + t.lane;

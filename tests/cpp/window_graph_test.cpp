@@ -46,6 +46,11 @@ struct SerialTeam {
     return 0;
   }
   bool any(bool b) const { return b; }
+  std::uint32_t uniform(std::uint32_t v) const { return v; }
+  std::uint64_t ballot(bool b) const { return b ? 1u : 0u; }
+  std::uint32_t below(std::uint64_t) const { return 0; }
+  std::uint32_t readlane(std::uint32_t v, std::uint32_t) const { return v; }
+  std::uint32_t min16(std::uint32_t v) const { return v; }
   std::uint32_t min_all(std::uint32_t v) const { return v; }
   std::uint32_t max_all(std::uint32_t v) const { return v; }
   std::uint32_t sum_all(std::uint32_t v) const { return v; }
