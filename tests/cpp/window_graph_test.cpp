@@ -2,21 +2,28 @@
 // graph: AddAlignment, TopologicalSort, the alignment kernels' graph block,
 // consensus and MSA), run with a one-lane team against the product's flat
 // host graph (PoaGraph) and the oracle, after every read of seeded random
-// windows in all nine scoring modes, with and without Phred weights.
+// windows in all nine scoring modes, with and without Phred weights.  Every
+// other read the topological sort also runs on a 4-lane team of threads in
+// lockstep, so that the lane-parallel steps of its walk (list chunks,
+// ballots, prefix counts, lane minima) are checked against the one-lane
+// result.
 //
 // Built and run by tests/test_window_graph.py:
-//   g++ -std=c++17 -O1 -I include -I spoa_amd/csrc window_graph_test.cpp
+//   g++ -std=c++17 -O2 -pthread -I include -I spoa_amd/csrc window_graph_test.cpp
 //       spoa_amd/csrc/poa_graph.cpp oracle/spoa_oracle.cpp
 
 #define __host__
 #define __device__
 #include "window_graph.hpp"
 
+#include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "poa_graph.hpp"
@@ -55,6 +62,106 @@ struct SerialTeam {
   std::uint32_t max_all(std::uint32_t v) const { return v; }
   std::uint32_t sum_all(std::uint32_t v) const { return v; }
 };
+
+// A team of L lanes as L threads in lockstep: every team operation is a
+// barrier-separated exchange, so the lanes run the same steps between
+// barriers as a wave's lanes run the same instructions.  Between two
+// barriers every lane writes the same values or distinct locations, which
+// is what the wave code relies on too.
+struct LaneShared {
+  explicit LaneShared(std::uint32_t n) : L(n) {}
+  std::uint32_t L;
+  std::atomic<std::uint32_t> arrived{0}, gen{0};
+  std::uint64_t buf[64] = {};
+  void barrier() {
+    const std::uint32_t g = gen.load(std::memory_order_acquire);
+    if (arrived.fetch_add(1, std::memory_order_acq_rel) + 1 == L) {
+      arrived.store(0, std::memory_order_relaxed);
+      gen.fetch_add(1, std::memory_order_acq_rel);
+    } else {
+      while (gen.load(std::memory_order_acquire) == g) std::this_thread::yield();
+    }
+  }
+};
+
+template <std::uint32_t L>
+struct LaneTeam {
+  std::uint32_t lane;
+  LaneShared* sh;
+  static constexpr std::uint32_t size = L;
+  void sync() const { sh->barrier(); }
+  // every lane's value, then a reduction of them (the same on every lane)
+  template <class F>
+  std::uint64_t all(std::uint64_t v, F f) const {
+    sh->buf[lane] = v;
+    sh->barrier();
+    const std::uint64_t x = f(sh->buf);
+    sh->barrier();
+    return x;
+  }
+  std::uint32_t scan(std::uint32_t v, std::uint32_t* total) const {
+    const std::uint64_t packed = all(v, [&](const std::uint64_t* b) {
+      std::uint64_t ex = 0, tot = 0;
+      for (std::uint32_t i = 0; i < L; ++i) {
+        if (i < lane) ex += b[i];
+        tot += b[i];
+      }
+      return ex | tot << 32;
+    });
+    *total = static_cast<std::uint32_t>(packed >> 32);
+    return static_cast<std::uint32_t>(packed);
+  }
+  bool any(bool b) const {
+    return all(b, [](const std::uint64_t* x) {
+             std::uint64_t a = 0;
+             for (std::uint32_t i = 0; i < L; ++i) a |= x[i];
+             return a;
+           }) != 0;
+  }
+  std::uint32_t uniform(std::uint32_t v) const {
+    return static_cast<std::uint32_t>(all(v, [](const std::uint64_t* x) { return x[0]; }));
+  }
+  std::uint64_t ballot(bool b) const {
+    return all(b, [](const std::uint64_t* x) {
+      std::uint64_t m = 0;
+      for (std::uint32_t i = 0; i < L; ++i) m |= (x[i] ? 1ull : 0ull) << i;
+      return m;
+    });
+  }
+  std::uint32_t below(std::uint64_t m) const {
+    return static_cast<std::uint32_t>(__builtin_popcountll(m & ((1ull << lane) - 1)));
+  }
+  std::uint32_t readlane(std::uint32_t v, std::uint32_t l) const {
+    return static_cast<std::uint32_t>(all(v, [&](const std::uint64_t* x) { return x[l]; }));
+  }
+  std::uint32_t reduce(std::uint32_t v, std::uint32_t n, int op) const {
+    return static_cast<std::uint32_t>(all(v, [&](const std::uint64_t* x) {
+      std::uint64_t a = x[0];
+      for (std::uint32_t i = 1; i < n; ++i) a = op == 0 ? std::min(a, x[i]) : op == 1 ? std::max(a, x[i]) : a + x[i];
+      return a;
+    }));
+  }
+  std::uint32_t min16(std::uint32_t v) const { return reduce(v, L < 16 ? L : 16, 0); }
+  std::uint32_t min_all(std::uint32_t v) const { return reduce(v, L, 0); }
+  std::uint32_t max_all(std::uint32_t v) const { return reduce(v, L, 1); }
+  std::uint32_t sum_all(std::uint32_t v) const { return reduce(v, L, 2); }
+};
+
+// The topological sort run by an L-lane team of threads.
+template <std::uint32_t L>
+static bool SortWithLanes(const WinView& gv, const SortSpace<std::uint32_t>& sp) {
+  LaneShared sh(L);
+  std::vector<std::thread> th;
+  std::vector<int> ok(L, 0);
+  for (std::uint32_t l = 0; l < L; ++l) {
+    th.emplace_back([&, l] { ok[l] = wg::TopologicalSort(LaneTeam<L>{l, &sh}, gv, sp) ? 1 : 0; });
+  }
+  for (auto& x : th) x.join();
+  for (std::uint32_t l = 0; l < L; ++l) {
+    if (!ok[l]) return false;
+  }
+  return true;
+}
 
 static int g_fail = 0;
 #define CHECK(c, ...)                          \
@@ -235,6 +342,23 @@ static void RunWindow(std::uint64_t seed, int mode, bool quality, bool small_sta
       if (i > 0) CHECK(static_cast<std::int32_t>(rows[4 * (i - 1) + 3]) == fcH[i], "row fcH %u", i);
     }
     CHECK(E == static_cast<std::uint32_t>(hv.pred_off[V] - hv.pred_off[0]), "E");
+    if (k % 2 == 0) {
+      // the sort again by a 4-lane team (a node's lists then span several
+      // lane chunks): the same ranks, and the same graph block (its first
+      // columns follow the sort's source distances)
+      const std::vector<std::uint32_t> block1(block.begin(), block.begin() + BlockWords(V, E));
+      std::vector<std::uint32_t> space(SortSpaceBytes(V, E, A, V + E + A + 1, nw, 4) / 4 + 2, 0);
+      auto sp4 = wg::CarveSortSpace<std::uint32_t>(space.data(), V, E, A, V + E + A + 1, nw);
+      CHECK(SortWithLanes<4>(gv, sp4), "4-lane sort overflow");
+      for (std::uint32_t i = 0; i < V; ++i) {
+        CHECK(gv.rank_to_node[i] == ranks[i], "4-lane rank %u: %u vs %u (seed %llu read %d)", i, gv.rank_to_node[i],
+              ranks[i], (unsigned long long)seed, k);
+        CHECK(gv.rank_of[ranks[i]] == i, "4-lane rank_of (seed %llu read %d)", (unsigned long long)seed, k);
+      }
+      wg::BuildBlock(t, gv, sp4, block.data(), P[0], sub, g, e, q, c);
+      CHECK(std::equal(block1.begin(), block1.end(), block.begin()), "4-lane graph block (seed %llu read %d)",
+            (unsigned long long)seed, k);
+    }
   }
   // consensus and MSA
   std::vector<std::uint32_t> cons(gv.hdr->V + 1), col(gv.hdr->V + 1);

@@ -7,7 +7,9 @@ the product's host graph (PoaGraph, itself pinned to the oracle) and the
 oracle: ranks, node ids, predecessor lists, sink flags, first columns by the
 reference's recurrence, consensus and MSA.  The 16-bit sort space is run
 with a three-entry stack so that its overflow path (a retry in 32-bit
-device scratch) is exercised too."""
+device scratch) is exercised too, and every other read the sort runs again
+on a 4-lane team of threads in lockstep (its lists then span several lane
+chunks), which must give the same ranks and graph block."""
 import os
 import subprocess
 
@@ -18,7 +20,7 @@ def test_window_graph_against_host_graph_and_oracle(tmp_path):
     exe = tmp_path / "window_graph_test"
     root = common.ROOT
     subprocess.check_call([
-        "g++", "-std=c++17", "-O2", "-I", os.path.join(root, "include"), "-I", os.path.join(root, "spoa_amd", "csrc"),
+        "g++", "-std=c++17", "-O2", "-pthread", "-I", os.path.join(root, "include"), "-I", os.path.join(root, "spoa_amd", "csrc"),
         os.path.join(root, "tests", "cpp", "window_graph_test.cpp"),
         os.path.join(root, "spoa_amd", "csrc", "poa_graph.cpp"),
         os.path.join(root, "oracle", "spoa_oracle.cpp"), "-o", str(exe)])
