@@ -96,6 +96,14 @@ std::vector<PoaGraph>& BatchContext::graphs(std::size_t n) {
   return graphs_;
 }
 
+void WindowResult::SetMsa(const std::vector<std::string>& rows) {
+  msa_rows = static_cast<std::uint32_t>(rows.size());
+  msa_width = rows.empty() ? 0u : static_cast<std::uint32_t>(rows[0].size());
+  msa.clear();
+  msa.reserve(static_cast<std::size_t>(msa_rows) * msa_width);
+  for (const auto& r : rows) msa += r;
+}
+
 std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats) {
   BatchContext ctx(opt);
   return RunBatched(ws, ctx, stats);
@@ -226,8 +234,9 @@ std::vector<WindowResult> RunResident(const WindowSet& ws, BatchContext& ctx, Ba
           const char* rows = nullptr;
           std::uint32_t nr = 0, width = 0;
           spoa_hip_windows_msa(p.win, wi, &rows, &nr, &width);
-          o.msa.resize(nr);
-          for (std::uint32_t k = 0; k < nr; ++k) o.msa[k].assign(rows + static_cast<std::size_t>(k) * width, width);
+          o.msa.assign(rows, static_cast<std::size_t>(nr) * width);
+          o.msa_rows = nr;
+          o.msa_width = width;
         }
         if (opt.strand) {
           const std::uint32_t n = ws.win_read_off[p.first + i + 1] - ws.win_read_off[p.first + i];
@@ -470,7 +479,7 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
   }
   pool.For(W, [&](std::size_t w) {
     out[w].consensus = graphs[w].GenerateConsensus();
-    if (opt.msa) out[w].msa = graphs[w].GenerateMultipleSequenceAlignment();
+    if (opt.msa) out[w].SetMsa(graphs[w].GenerateMultipleSequenceAlignment());
   });
   st.finish_ms = Ms(tf, Clock::now());
   st.wall_ms = Ms(t0, Clock::now());
@@ -538,7 +547,7 @@ void PerCallWindows(const WindowSet& ws, const BatchOptions& opt, std::uint32_t 
     }
     const auto tf = Clock::now();
     out[w].consensus = graph.GenerateConsensus();
-    if (opt.msa) out[w].msa = graph.GenerateMultipleSequenceAlignment();
+    if (opt.msa) out[w].SetMsa(graph.GenerateMultipleSequenceAlignment());
     st.finish_ms += Ms(tf, Clock::now());
   }
 }

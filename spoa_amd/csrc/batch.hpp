@@ -50,8 +50,15 @@ struct BatchOptions {
 
 struct WindowResult {
   std::string consensus;
-  std::vector<std::string> msa;
+  // the multiple sequence alignment as one block: msa_rows rows of msa_width
+  // characters each, one after another (every read in order, then the
+  // consensus row when asked for)
+  std::string msa;
+  std::uint32_t msa_rows = 0, msa_width = 0;
   std::vector<std::uint8_t> reversed;  // strand mode: read kept reverse-complemented
+
+  // the rows of a graph's multiple sequence alignment, into the block
+  void SetMsa(const std::vector<std::string>& rows);
 };
 
 struct BatchStats {

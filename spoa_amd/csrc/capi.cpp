@@ -506,12 +506,11 @@ int64_t spoa_batch_consensus_all(void* h, char* out, uint64_t cap, uint64_t* off
 int spoa_batch_msa(void* h, uint32_t w, char* out, uint64_t cap, uint32_t* rows, uint32_t* cols) {
   const auto& r = static_cast<BatchHandle*>(h)->results;
   if (w >= r.size()) return -1;
-  const auto& m = r[w].msa;
-  *rows = static_cast<uint32_t>(m.size());
-  *cols = m.empty() ? 0 : static_cast<uint32_t>(m[0].size());
-  const uint64_t need = static_cast<uint64_t>(*rows) * *cols;
-  if (need > cap) return -2;
-  for (std::size_t k = 0; k < m.size(); ++k) std::memcpy(out + k * (*cols), m[k].data(), *cols);
+  const auto& o = r[w];
+  *rows = o.msa_rows;
+  *cols = o.msa_width;
+  if (o.msa.size() > cap) return -2;
+  if (!o.msa.empty()) std::memcpy(out, o.msa.data(), o.msa.size());
   return 0;
 }
 
