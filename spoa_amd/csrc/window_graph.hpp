@@ -653,11 +653,15 @@ __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const 
   auto min_lanes = [&](uint32_t v, uint32_t cnt) { return cnt <= 16 ? t.min16(v) : U(t.min_all(v)); };
   for (uint32_t base = 0; base < Vu && ok; base += 8) {
     uint32_t j = 0;
+    uint64_t wu = 0;
+    bool word = false;  // wu holds the marks of nodes base..base+7
     while (ok && j < 8) {
       // the next root: the first unmarked node at or after base + j
-      const uint64_t w = *reinterpret_cast<const uint64_t*>(mk + base);
-      const uint64_t wu = static_cast<uint64_t>(U(static_cast<uint32_t>(w >> 32))) << 32 |
-                          U(static_cast<uint32_t>(w));
+      if (!word) {
+        const uint64_t w = *reinterpret_cast<const uint64_t*>(mk + base);
+        wu = static_cast<uint64_t>(U(static_cast<uint32_t>(w >> 32))) << 32 | U(static_cast<uint32_t>(w));
+        word = true;
+      }
       uint64_t z = ZeroBytes(wu) & (~0ull << (8 * j));
       if (Vu - base < 8) z &= (1ull << (8 * (Vu - base))) - 1;
       if (z == 0) break;
@@ -670,6 +674,7 @@ __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const 
       }
       sp.stack[0] = static_cast<Idx>(cur);
       top = 1;
+      bool single = true;  // the root finished at its first visit, pushing nothing
       while (top > 0) {
         const uint32_t m = U(mk[cur]);
         const uint32_t i0 = U(sp.in_off[cur]), i1 = U(sp.in_off[cur + 1]);
@@ -721,6 +726,7 @@ __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const 
           if (pushed) {
             mk[cur] = static_cast<uint8_t>(m | 1);
             cur = last;
+            single = false;
             continue;
           }
         } else {
@@ -747,6 +753,12 @@ __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const 
         }
         --top;
         cur = below;
+      }
+      // a root that finished alone changed no mark but its own
+      if (single) {
+        wu |= 2ull << (8 * (j - 1));
+      } else {
+        word = false;
       }
     }
   }
