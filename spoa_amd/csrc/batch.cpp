@@ -96,14 +96,6 @@ std::vector<PoaGraph>& BatchContext::graphs(std::size_t n) {
   return graphs_;
 }
 
-void WindowResult::SetMsa(const std::vector<std::string>& rows) {
-  msa_rows = static_cast<std::uint32_t>(rows.size());
-  msa_width = rows.empty() ? 0u : static_cast<std::uint32_t>(rows[0].size());
-  msa.clear();
-  msa.reserve(static_cast<std::size_t>(msa_rows) * msa_width);
-  for (const auto& r : rows) msa += r;
-}
-
 std::vector<WindowResult> RunBatched(const WindowSet& ws, const BatchOptions& opt, BatchStats* stats) {
   BatchContext ctx(opt);
   return RunBatched(ws, ctx, stats);
@@ -479,7 +471,7 @@ std::vector<WindowResult> RunBatched(const WindowSet& ws, BatchContext& ctx, Bat
   }
   pool.For(W, [&](std::size_t w) {
     out[w].consensus = graphs[w].GenerateConsensus();
-    if (opt.msa) out[w].SetMsa(graphs[w].GenerateMultipleSequenceAlignment());
+    if (opt.msa) graphs[w].MultipleSequenceAlignmentBlock(false, &out[w].msa, &out[w].msa_rows, &out[w].msa_width);
   });
   st.finish_ms = Ms(tf, Clock::now());
   st.wall_ms = Ms(t0, Clock::now());
@@ -547,7 +539,15 @@ void PerCallWindows(const WindowSet& ws, const BatchOptions& opt, std::uint32_t 
     }
     const auto tf = Clock::now();
     out[w].consensus = graph.GenerateConsensus();
-    if (opt.msa) out[w].SetMsa(graph.GenerateMultipleSequenceAlignment());
+    if (opt.msa) {
+      const std::vector<std::string> rows = graph.GenerateMultipleSequenceAlignment();
+      WindowResult& o = out[w];
+      o.msa_rows = static_cast<std::uint32_t>(rows.size());
+      o.msa_width = rows.empty() ? 0u : static_cast<std::uint32_t>(rows[0].size());
+      o.msa.clear();
+      o.msa.reserve(static_cast<std::size_t>(o.msa_rows) * o.msa_width);
+      for (const auto& row : rows) o.msa += row;
+    }
     st.finish_ms += Ms(tf, Clock::now());
   }
 }

@@ -56,9 +56,6 @@ struct WindowResult {
   std::string msa;
   std::uint32_t msa_rows = 0, msa_width = 0;
   std::vector<std::uint8_t> reversed;  // strand mode: read kept reverse-complemented
-
-  // the rows of a graph's multiple sequence alignment, into the block
-  void SetMsa(const std::vector<std::string>& rows);
 };
 
 struct BatchStats {

@@ -324,25 +324,36 @@ std::vector<std::uint32_t> PoaGraph::MsaColumns(std::uint32_t* width) const {
 }
 
 // src/graph.cpp:339-366
-std::vector<std::string> PoaGraph::GenerateMultipleSequenceAlignment(bool include_consensus) {
-  std::uint32_t width = 0;
-  const std::vector<std::uint32_t> col = MsaColumns(&width);
-  std::vector<std::string> rows;
-  rows.reserve(num_sequences() + (include_consensus ? 1 : 0));
+void PoaGraph::MultipleSequenceAlignmentBlock(bool include_consensus, std::string* block, std::uint32_t* n_rows,
+                                              std::uint32_t* width) {
+  std::uint32_t w = 0;
+  const std::vector<std::uint32_t> col = MsaColumns(&w);
+  const std::uint32_t n = num_sequences() + (include_consensus ? 1 : 0);
+  block->assign(static_cast<std::size_t>(n) * w, '-');
+  char* const out = block->empty() ? nullptr : &(*block)[0];
   for (std::uint32_t s = 0; s < num_sequences(); ++s) {
-    std::string row(width, '-');
+    char* const row = out + static_cast<std::size_t>(s) * w;
     for (std::uint32_t k = path_off_[s]; k < path_off_[s + 1]; ++k) {
       const std::uint32_t v = path_[k];
       row[col[v]] = static_cast<char>(decoder_[code_[v]]);
     }
-    rows.push_back(std::move(row));
   }
   if (include_consensus) {
     TraverseHeaviestBundle();
-    std::string row(width, '-');
+    char* const row = out + static_cast<std::size_t>(num_sequences()) * w;
     for (std::uint32_t v : consensus_) row[col[v]] = static_cast<char>(decoder_[code_[v]]);
-    rows.push_back(std::move(row));
   }
+  *n_rows = n;
+  *width = w;
+}
+
+std::vector<std::string> PoaGraph::GenerateMultipleSequenceAlignment(bool include_consensus) {
+  std::string block;
+  std::uint32_t n = 0, w = 0;
+  MultipleSequenceAlignmentBlock(include_consensus, &block, &n, &w);
+  std::vector<std::string> rows;
+  rows.reserve(n);
+  for (std::uint32_t i = 0; i < n; ++i) rows.emplace_back(block, static_cast<std::size_t>(i) * w, w);
   return rows;
 }
 

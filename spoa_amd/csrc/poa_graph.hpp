@@ -92,6 +92,10 @@ class PoaGraph {
   std::string GenerateConsensus();
   // Graph::GenerateMultipleSequenceAlignment (src/graph.cpp:339-366)
   std::vector<std::string> GenerateMultipleSequenceAlignment(bool include_consensus = false);
+  // the same rows one after another in one block: n_rows rows of width
+  // characters
+  void MultipleSequenceAlignmentBlock(bool include_consensus, std::string* block, std::uint32_t* n_rows,
+                                      std::uint32_t* width);
 
   const std::vector<std::uint32_t>& rank_to_node() const { return rank_to_node_; }
 
