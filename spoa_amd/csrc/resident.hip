@@ -402,9 +402,14 @@ char Complement(char b) {
 }
 
 // LDS of the merge launch: the largest 16-bit sort space among its windows
-// (their sizes after the merge bounded by the read) with a stack of 2,048,
-// at most kMergeLdsMax; larger windows sort in device memory.
+// (their sizes after the merge bounded by the read) with a stack of
+// kMergeStack entries, at most kMergeLdsMax; larger windows, and a sort
+// whose stack outgrows what is left, sort in device memory.  (The deepest
+// stacks measured: 327 entries on overlap windows of 500 bp, whose reads'
+// overhangs become chains of new nodes; about 10 on the others.)  The LDS
+// a workgroup takes sets how many windows sort side by side on a CU.
 constexpr uint64_t kMergeLdsMax = 160 * 1024;
+constexpr uint64_t kMergeStack = 512;
 uint32_t MergeLds(const spoa_hip_windows* X, uint32_t r, bool nw) {
   uint64_t best = 4096;
   for (uint32_t w : X->live) {
@@ -412,7 +417,7 @@ uint32_t MergeLds(const spoa_hip_windows* X, uint32_t r, bool nw) {
     const uint64_t len = X->read_len[X->read_first[w] + r];
     const uint64_t V = s.V + len, E = s.E + len, A = s.A + 2 * len;
     if (V >= 65535 || E >= 65535 || A >= 65535) continue;
-    const uint64_t b = wg::SortSpaceBytes(V, E, A, 2048, nw, 2) + 64;
+    const uint64_t b = wg::SortSpaceBytes(V, E, A, kMergeStack, nw, 2) + 64;
     best = std::max(best, std::min<uint64_t>(b, kMergeLdsMax));
   }
   return static_cast<uint32_t>((best + 1023) / 1024 * 1024);

@@ -91,11 +91,11 @@ struct WinLayout {
 // offsets and tails by node id, aligned-list offsets and entries, the stack
 // (every push follows an in-edge or an aligned entry of a node visited once,
 // or starts a root), for global alignments the source distance of every
-// node, the rank order, a sink, then the marks (a byte each) at an 8-byte
-// boundary and their sink.
+// node, a sink, then the marks (a byte each) at an 8-byte boundary and
+// their sink.
 __host__ __device__ inline uint64_t SortScratchWords(uint64_t V, uint64_t E) {
   const uint64_t A = V * kMaxAligned;
-  return (V + 1) + E + (V + 1) + A + (V + E + A + 1) + V + V + kSortSink + 1 + (V + 7) / 8 * 2 + kSortSink / 4;
+  return (V + 1) + E + (V + 1) + A + (V + E + A + 1) + V + kSortSink + 1 + (V + 7) / 8 * 2 + kSortSink / 4;
 }
 
 __host__ __device__ inline WinLayout MakeLayout(const WinCaps& c) {
@@ -527,7 +527,6 @@ struct SortSpace {
   Idx* stack;    // [K]
   uint32_t K;
   Idx* dist;       // [V] source distance by node id (global alignments only)
-  Idx* r2n;        // [V] rank order
   Idx* sink;       // [kSortSink]
   uint8_t* marks;  // [V rounded up to 8], 8-byte aligned: 1 visited, 2 finished,
                    // 4 ranked with its group's first member
@@ -536,7 +535,7 @@ struct SortSpace {
 
 // Words (of Idx) of a SortSpace's index arrays.
 __host__ __device__ inline uint64_t SortSpaceElems(uint64_t V, uint64_t E, uint64_t A, uint64_t K, bool nw) {
-  return (V + 1) + E + (V + 1) + A + K + (nw ? V : 0) + V + kSortSink;
+  return (V + 1) + E + (V + 1) + A + K + (nw ? V : 0) + kSortSink;
 }
 
 // Bytes of a SortSpace from an 8-byte aligned base: the index arrays, then
@@ -564,8 +563,6 @@ __host__ __device__ inline SortSpace<Idx> CarveSortSpace(void* base, uint32_t V,
   s.K = K;
   s.dist = nw ? p : nullptr;
   p += nw ? V : 0;
-  s.r2n = p;
-  p += V;
   s.sink = p;
   const uint64_t idx_bytes = (SortSpaceElems(V, E, A, K, nw) * sizeof(Idx) + 7) / 8 * 8;
   s.marks = static_cast<uint8_t*>(base) + idx_bytes;
@@ -598,9 +595,9 @@ __host__ __device__ inline uint64_t ZeroBytes(uint64_t w) {
 // aligned nodes (the order the reference pushes them in), reads it and its
 // mark, and the unfinished ones go onto the stack in that order through one
 // ballot and a prefix count.  Lanes with nothing to write write to a sink
-// slot of their own instead of branching.  The rank order is kept in `sp`
-// and written out team-parallel at the end.  Returns false when the stack
-// of `sp` overflows.
+// slot of their own instead of branching.  The rank order is written to the
+// window as it forms, its inverse team-parallel at the end.  Returns false
+// when the stack of `sp` overflows.
 template <class Team, typename Idx>
 __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const SortSpace<Idx>& sp) {
   const uint32_t V = g.hdr->V, E = g.hdr->E;
@@ -640,6 +637,11 @@ __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const 
   t.sync();
   const bool nw = sp.dist != nullptr;
   uint8_t* const mk = sp.marks;
+  // the rank order goes straight to the window; writes nothing reads go to
+  // a sink of 32-bit slots: the window's scratch when the sort space is
+  // elsewhere, else the sort space's own sink
+  uint32_t* const r2n = g.rank_to_node;
+  uint32_t* const rsink = sizeof(Idx) == 4 ? reinterpret_cast<uint32_t*>(sp.sink) : g.scratch;
   auto U = [&](uint32_t x) { return t.uniform(x); };
   const uint32_t Vu = U(V), K = U(sp.K), lane = t.lane;
   uint32_t r = 0, top = 0;
@@ -715,7 +717,7 @@ __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const 
               last = t.readlane(val, 63u - static_cast<uint32_t>(__builtin_clzll(pm)));
               pushed = true;
             }
-            if (!ign) *(valid && qq >= nt ? sp.r2n + r + 1 + (qq - nt) : sp.sink + lane) = static_cast<Idx>(val);
+            if (!ign) *(valid && qq >= nt ? r2n + r + 1 + (qq - nt) : rsink + lane) = val;
             if (nw && c < nt) {
               const uint32_t dv = sp.dist[val];
               const uint32_t dm = min_lanes(valid && qq < nt ? dv : kNil, nt - c);
@@ -737,7 +739,7 @@ __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const 
             const bool valid = q < n;
             const uint32_t qq = valid ? q : c;
             const uint32_t val = entry(qq, i0, nt, a0);
-            if (!ign) *(valid && qq >= nt ? sp.r2n + r + 1 + (qq - nt) : sp.sink + lane) = static_cast<Idx>(val);
+            if (!ign) *(valid && qq >= nt ? r2n + r + 1 + (qq - nt) : rsink + lane) = val;
             if (nw && c < nt) {
               const uint32_t dv = sp.dist[val];
               const uint32_t dm = min_lanes(valid && qq < nt ? dv : kNil, nt - c);
@@ -748,7 +750,7 @@ __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const 
         mk[cur] = static_cast<uint8_t>(m | 2);
         if (nw) sp.dist[cur] = static_cast<Idx>(nt > 0 ? dmin + 1 : 0u);
         if (!ign) {
-          sp.r2n[r] = static_cast<Idx>(cur);
+          r2n[r] = cur;
           r += 1 + (a1 - a0);
         }
         --top;
@@ -763,12 +765,8 @@ __host__ __device__ bool TopologicalSort(const Team& t, const WinView& g, const 
     }
   }
   t.sync();
-  // the rank order out to the window (rank_to_node and its inverse)
-  for (uint32_t q = t.lane; q < V; q += t.size) {
-    const uint32_t v = sp.r2n[q];
-    g.rank_to_node[q] = v;
-    g.rank_of[v] = q;
-  }
+  // the inverse of the rank order
+  for (uint32_t q = t.lane; q < V; q += t.size) g.rank_of[r2n[q]] = q;
   t.sync();
   return t.any(!ok) ? false : true;
 }
