@@ -711,9 +711,13 @@ std::vector<TuneResult> Autotune(const WindowSet& ws, BatchContext& ctx, std::ui
           set(width, cd.first, cd.second);
           BatchStats st;
           (void)RunBatched(ws, ctx, &st);
-          const double ms = width == 16 ? st.kernel_ms16 : st.kernel_ms32;
+          // An int32-width launch runs beside the int16 launches and takes
+          // part of the device from them, so its shapes are judged by both
+          // widths' launch time, which is what a step pays.
+          const double own = width == 16 ? st.kernel_ms16 : st.kernel_ms32;
+          const double ms = width == 16 ? own : st.kernel_ms16 + st.kernel_ms32;
           tr.tried.push_back({cd, ms});
-          if (ms <= 0) continue;  // no job of this width
+          if (own <= 0) continue;  // no job of this width
           if (best < 0 || ms < best) {
             best = ms;
             tr.k = cd.first;

@@ -124,7 +124,8 @@ std::vector<WindowResult> RunResident(const WindowSet& ws, BatchContext& ctx, Ba
 struct TuneResult {
   int width = 0;
   int k = 0, waves = 0;
-  double kernel_ms = 0;      // of the chosen configuration (0: from the cache)
+  double kernel_ms = 0;      // of the chosen configuration (0: from the cache); for the
+                             // int32 width, both widths' launches (they share the device)
   bool cached = false;
   bool whole_batch = false;  // the candidates ran the whole batch (no job of this width in the first rounds)
   std::vector<std::pair<std::pair<int, int>, double>> tried;  // ((k, waves), kernel ms)
