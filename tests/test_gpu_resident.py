@@ -7,7 +7,8 @@ the device.
   types, all three gap models, strand mode, Phred weights).
 * Whole runs equal the compiled reference's vectors and the oracle
   (consensus, MSA, kept strands), and the host-merged batched form.
-* A window too large for the LDS sort space sorts in device memory."""
+* A window too large for the LDS sort space, or whose walk outgrows the
+  LDS stack, sorts in device memory."""
 import ctypes
 import gzip
 import hashlib
@@ -195,6 +196,25 @@ def test_resident_large_window_sorts_in_device_memory():
     res = spoa_amd.run_windows(windows, *params, mode="resident", msa=True)
     assert res.consensus == host.consensus
     assert res.msa == host.msa
+
+
+def test_resident_deep_stack_sorts_in_device_memory():
+    """An overlap read with a 1,500-base overhang becomes a chain of new
+    nodes that the depth-first walk descends in one go; the later reads are
+    50 bases, so the LDS room of their rounds (sized by the read) is smaller
+    than that stack: the 16-bit sort reports the overflow and the window
+    sorts again in its device scratch.  Consensus and MSA equal the
+    oracle's; a neighbouring window without the chain too."""
+    rng = random.Random(5)
+    templ = "".join(rng.choice("ACGT") for _ in range(300))
+    over = "".join(rng.choice("ACGT") for _ in range(1500))
+    deep = [templ, over + templ] + [templ[i:i + 50] for i in (10, 100, 170, 240)]
+    plain = [_mutate(rng, templ, 0.1) for _ in range(6)]
+    params = (ol.OV, 5, -4, -8, -6, -10, -4)
+    res = spoa_amd.run_windows([deep, plain], *params, mode="resident", msa=True)
+    for w, reads in enumerate((deep, plain)):
+        cons, msa, _ = ol.run_window(ol.oracle(), params, reads)
+        assert res.consensus[w] == cons and res.msa[w] == msa, w
 
 
 def test_context_reuses_buffers_across_window_sets():
